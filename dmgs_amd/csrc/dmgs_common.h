@@ -1,0 +1,103 @@
+// Shared device helpers for the gfx950 rasterizer kernels.
+//
+// Conventions (the reference rasterizer's, diff-gaussian-rasterization [U],
+// called from gaussian_renderer/__init__.py:86-94):
+//   * view / projection matrices are 16 floats in the order the reference
+//     stores world_view_transform / full_proj_transform (row-vector
+//     convention), i.e. p_view.x = m[0] x + m[4] y + m[8] z + m[12];
+//   * tiles are 16 x 16 pixels; tile id = ty * grid_x + tx;
+//   * covariances are 6-vectors (xx, xy, xz, yy, yz, zz).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define DMGS_BLOCK_X 16
+#define DMGS_BLOCK_Y 16
+#define DMGS_BLOCK_SIZE (DMGS_BLOCK_X * DMGS_BLOCK_Y)
+#define DMGS_WAVE 64
+
+// Per-Gaussian render record ("splat"): three float4 per Gaussian.
+//   s0 = (pixel x, pixel y, conic a, conic b)
+//   s1 = (conic c, opacity, red, green)
+//   s2 = (blue, view depth, radius as int bits, 0)
+// One 48-byte gather feeds both blend kernels.
+#define DMGS_SPLAT_F4 3
+
+// Per-Gaussian accumulator written by the blend backward with atomics:
+// 16 floats (64 B, one cache line) per Gaussian.
+//   [0] dL/dmean2D.x (NDC)  [1] dL/dmean2D.y  [2] dL/dconic.a  [3] dL/dconic.b (half)
+//   [4] dL/dconic.c  [5] dL/dopacity  [6..8] dL/dcolour rgb
+#define DMGS_GACC_STRIDE 16
+#define DMGS_GACC_USED 9
+
+namespace dmgs {
+
+__device__ __forceinline__ float3 xform_point_4x3(const float3 p, const float* m) {
+  return make_float3(m[0] * p.x + m[4] * p.y + m[8] * p.z + m[12],
+                     m[1] * p.x + m[5] * p.y + m[9] * p.z + m[13],
+                     m[2] * p.x + m[6] * p.y + m[10] * p.z + m[14]);
+}
+
+__device__ __forceinline__ float4 xform_point_4x4(const float3 p, const float* m) {
+  return make_float4(m[0] * p.x + m[4] * p.y + m[8] * p.z + m[12],
+                     m[1] * p.x + m[5] * p.y + m[9] * p.z + m[13],
+                     m[2] * p.x + m[6] * p.y + m[10] * p.z + m[14],
+                     m[3] * p.x + m[7] * p.y + m[11] * p.z + m[15]);
+}
+
+__device__ __forceinline__ float ndc_to_pix(float v, int S) { return ((v + 1.0f) * S - 1.0f) * 0.5f; }
+
+// Screen rectangle of tiles touched by a splat of integer radius r centred
+// at p: [min, max) per axis, clamped to the grid.  The float operation order
+// is fixed so that the host oracle can reproduce it exactly.
+__device__ __forceinline__ void tile_rect(float2 p, int r, int gx, int gy, int2& rmin, int2& rmax) {
+  const float fr = (float)r;
+  const float b = (float)DMGS_BLOCK_X;
+  rmin.x = min(gx, max(0, (int)((p.x - fr) / b)));
+  rmin.y = min(gy, max(0, (int)((p.y - fr) / b)));
+  rmax.x = min(gx, max(0, (int)((((p.x + fr) + b) - 1.0f) / b)));
+  rmax.y = min(gy, max(0, (int)((((p.y + fr) + b) - 1.0f) / b)));
+}
+
+// --- wave64 reductions --------------------------------------------------
+// Sum over the 16 lanes of each DPP row; every lane of the row gets the sum.
+__device__ __forceinline__ float row16_sum(float v) {
+  int x = __float_as_int(v);
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true));   // quad_perm 1,0,3,2
+  x = __float_as_int(v);
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true));   // quad_perm 2,3,0,1
+  x = __float_as_int(v);
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, true));  // row_ror 4
+  x = __float_as_int(v);
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, true));  // row_ror 8
+  return v;
+}
+
+// Wave-uniform sum of v over all 64 lanes (all lanes must be active).
+__device__ __forceinline__ float wave_sum(float v) {
+  v = row16_sum(v);
+  const int x = __float_as_int(v);
+  return (__int_as_float(__builtin_amdgcn_readlane(x, 0)) + __int_as_float(__builtin_amdgcn_readlane(x, 16))) +
+         (__int_as_float(__builtin_amdgcn_readlane(x, 32)) + __int_as_float(__builtin_amdgcn_readlane(x, 48)));
+}
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const unsigned l = __lane_id();
+  return (l == 0) ? 0ull : (~0ull >> (64 - l));
+}
+
+// Inclusive prefix sum across the wave (uint32).
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  const unsigned l = __lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(v, off, 64);
+    if (l >= (unsigned)off) v += o;
+  }
+  return v;
+}
+
+}  // namespace dmgs

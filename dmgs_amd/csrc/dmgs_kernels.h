@@ -1,0 +1,104 @@
+// Internal kernel argument structs and launchers (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace dmgs {
+
+struct PreprocessArgs {
+  int P;
+  int W, H, grid_x, grid_y;
+  float tanfovx, tanfovy, focal_x, focal_y, scale_modifier;
+  int sh_degree, sh_coeffs;
+  const float* means3D;
+  const float* scales;
+  const float* rotations;
+  const float* opacities;
+  const float* shs;
+  const float* colors_precomp;
+  const float* cov3D_precomp;
+  const float* viewmatrix;
+  const float* projmatrix;
+  const float* campos;
+  // outputs
+  int32_t* radii;
+  float4* splats;
+  uint32_t* tiles_touched;
+  uint32_t* depth_keys;
+  uint32_t* order;
+};
+
+void launch_preprocess_fwd(const PreprocessArgs& a, hipStream_t s);
+void launch_mark_visible(int P, const float* means3D, const float* viewmatrix, uint8_t* present, hipStream_t s);
+
+// ---- scan / sort / binning ----
+// Inclusive scan of n uint32 values; scratch must hold scan_scratch_words(n) words.
+size_t scan_scratch_words(size_t n);
+void launch_inclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch, hipStream_t s);
+
+// Stable LSD radix sort of (key, value) pairs over bits [0, bits), in passes of
+// at most 8 bits.  Data starts in (k0, v0); (k1, v1) is the ping-pong buffer.
+// The number of passes is always even, so the result ends in (k0, v0).
+size_t radix_scratch_words(size_t n);
+void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
+                      uint32_t* scratch, hipStream_t s);
+
+// tiles_touched gathered into depth order, ready for the scan.
+void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int n, hipStream_t s);
+
+// Emit (tile, gaussian) pairs in depth order.
+// The splat record carries the integer radius (0 = culled) in s2.z.
+void launch_duplicate(int P, const uint32_t* order, const uint32_t* offsets, const float4* splats, int grid_x,
+                      int grid_y, uint32_t* tile_keys, uint32_t* values, uint32_t K, hipStream_t s);
+void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s);
+
+// ---- blend ----
+struct RenderArgs {
+  int W, H, grid_x, grid_y;
+  uint32_t P, K;  // Gaussian count and list length (guards for the gathers)
+  const uint2* ranges;
+  const uint32_t* point_list;
+  const float4* splats;
+  const float* bg;
+  float* out_color;      // (3, H, W)
+  float* final_T;        // (H, W)
+  uint32_t* n_contrib;   // (H, W)
+  // backward only
+  const float* dL_dpix;  // (3, H, W)
+  float* gacc;           // (P, DMGS_GACC_STRIDE)
+};
+void launch_render_fwd(const RenderArgs& a, hipStream_t s);
+void launch_render_bwd(const RenderArgs& a, hipStream_t s);
+
+// ---- geometry backward ----
+struct GeometryBwdArgs {
+  int P;
+  int W, H;
+  float tanfovx, tanfovy, focal_x, focal_y, scale_modifier;
+  int sh_degree, sh_coeffs;
+  const float* means3D;
+  const float* scales;
+  const float* rotations;
+  const float* shs;
+  const float* cov3D_precomp;
+  const float* viewmatrix;
+  const float* projmatrix;
+  const float* campos;
+  const int32_t* radii;
+  const float* gacc;
+  // outputs (each may be null when the caller does not want it)
+  float* dL_dmeans2D;    // (P, 3)
+  float* dL_dcolors;     // (P, 3)
+  float* dL_dopacity;    // (P, 1)
+  float* dL_dconic;      // (P, 4)
+  float* dL_dmeans3D;    // (P, 3)
+  float* dL_dcov3D;      // (P, 6)
+  float* dL_dsh;         // (P, M, 3)
+  float* dL_dscales;     // (P, 3)
+  float* dL_drotations;  // (P, 4)
+};
+void launch_geometry_bwd(const GeometryBwdArgs& a, hipStream_t s);
+
+}  // namespace dmgs

@@ -1,0 +1,136 @@
+/*
+ * dmgs.h — C-ABI of the MI355X (gfx950) Gaussian rasterizer and mesh binding.
+ *
+ * Plain pointers and sizes only: every tensor argument is a device pointer to
+ * contiguous float32 / int32 / uint32 memory that the caller owns, and every
+ * call is enqueued on the caller's hipStream_t.  The library allocates
+ * nothing: each workspace is sized by a *_workspace_size() query and passed
+ * in as raw device memory (two-phase, like the reference's resize callbacks).
+ *
+ * Every entry point returns 0 on success, DMGS_EINVAL (-1) for an invalid
+ * argument (nothing is launched) or DMGS_EHIP (-2) when a HIP call or launch
+ * failed; dmgs_last_error() then returns a message (per calling thread).
+ *
+ * Reference interface being replaced: the pybind module `_C` of the
+ * un-vendored submodule diff-gaussian-rasterization (reference .gitmodules:4-6),
+ * reached from gaussian_renderer/__init__.py:14,86-94 through
+ * GaussianRasterizer / _RasterizeGaussians.  Its Python entry points map as:
+ *   _C.rasterize_gaussians          -> dmgs_forward_geometry + host read of the
+ *                                      count + dmgs_forward_render
+ *   _C.rasterize_gaussians_backward -> dmgs_backward
+ *   _C.mark_visible                 -> dmgs_mark_visible
+ * The mesh binding replaces the PyTorch code of
+ * scene/gaussian_geo_model_finetune.py:396-482 and
+ * scene/gaussian_geo_model_mlp_flex.py:258-385.
+ */
+#ifndef DMGS_H
+#define DMGS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* dmgs_stream_t; /* == hipStream_t */
+
+#define DMGS_OK 0
+#define DMGS_EINVAL (-1)
+#define DMGS_EHIP (-2)
+
+/* Mirrors GaussianRasterizationSettings (gaussian_renderer/__init__.py:36-49). */
+typedef struct dmgs_settings {
+  int32_t image_width;
+  int32_t image_height;
+  float tanfovx;
+  float tanfovy;
+  float scale_modifier;
+  int32_t sh_degree;   /* active SH degree D */
+  int32_t sh_coeffs;   /* M: SH coefficients per channel in `shs` (0 with precomputed colours) */
+  int32_t prefiltered; /* accepted for API parity; culling is identical either way */
+  int32_t debug;       /* synchronise and check for errors after every kernel */
+  const float* viewmatrix; /* device, 16 floats, reference storage order (world_view_transform) */
+  const float* projmatrix; /* device, 16 floats (full_proj_transform) */
+  const float* campos;     /* device, 3 floats */
+  const float* bg;         /* device, 3 floats */
+} dmgs_settings;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* dmgs_last_error(void);
+/* Library version string. */
+const char* dmgs_version(void);
+
+/* ---- workspace sizing (bytes) ------------------------------------------- */
+/* Per-Gaussian forward state kept until backward (the reference geomBuffer). */
+size_t dmgs_geometry_workspace_size(int64_t P);
+/* Per-(Gaussian, tile) sorted lists kept until backward (binningBuffer). */
+size_t dmgs_binning_workspace_size(int64_t K);
+/* Per-pixel and per-tile state kept until backward (imgBuffer). */
+size_t dmgs_image_workspace_size(int32_t width, int32_t height);
+/* Per-Gaussian gradient accumulator used only inside dmgs_backward. */
+size_t dmgs_backward_workspace_size(int64_t P);
+/* Byte offsets of the sub-buffers inside the workspaces, for inspection:
+ * geometry: [0] splats (P x 12 f32), [1] depth keys (P u32), [2] depth order (P u32),
+ *           [3] tiles touched (P u32), [4] inclusive offsets in depth order (P u32)
+ * binning:  [5] tile ids, sorted (K u32), [6] Gaussian ids, sorted (K u32)
+ * image:    [7] ranges (tiles x 2 u32), [8] final T (H*W f32), [9] n_contrib (H*W u32) */
+int dmgs_workspace_offsets(int64_t P, int64_t K, int32_t width, int32_t height, size_t offsets[10]);
+
+/* ---- forward -------------------------------------------------------------- */
+/* Preprocess every Gaussian (cull, project, EWA conic, radius, tile count,
+ * SH -> RGB), sort the Gaussians by depth and prefix-sum their tile counts.
+ * Exactly one of {shs, colors_precomp} and one of {(scales, rotations),
+ * cov3D_precomp} must be non-null.  radii: (P) int32 output.  The number of
+ * (Gaussian, tile) instances K is written to *num_rendered_dev (device u32).
+ * Replaces the first half of _C.rasterize_gaussians [U]. */
+int dmgs_forward_geometry(const dmgs_settings* s, int64_t P, const float* means3D, const float* opacities,
+                          const float* shs, const float* colors_precomp, const float* scales,
+                          const float* rotations, const float* cov3D_precomp, int32_t* radii, void* geom_ws,
+                          uint32_t* num_rendered_dev, dmgs_stream_t stream);
+
+/* Duplicate into (tile, Gaussian) instances, sort by tile, find per-tile
+ * ranges and blend front to back.  K must be the count written by
+ * dmgs_forward_geometry.  out_color: (3, H, W) float32.
+ * Replaces the second half of _C.rasterize_gaussians [U]. */
+int dmgs_forward_render(const dmgs_settings* s, int64_t P, int64_t K, void* geom_ws, void* binning_ws,
+                        void* image_ws, float* out_color, dmgs_stream_t stream);
+
+/* ---- backward ------------------------------------------------------------- */
+/* Reverse-traversal blend backward + fused cov2D / projection / SH / cov3D
+ * backward.  Inputs are the forward's inputs and workspaces; dL_dout is
+ * (3, H, W).  Outputs (caller-allocated, fully overwritten):
+ *   dL_dmeans2D (P,3) NDC units, z = 0; dL_dcolors (P,3); dL_dopacity (P,1);
+ *   dL_dconic (P,4) (.z = 0); dL_dmeans3D (P,3); dL_dcov3D (P,6);
+ *   dL_dsh (P,M,3) (null iff shs is null); dL_dscales (P,3), dL_drotations (P,4)
+ *   (null iff cov3D_precomp is given).
+ * Replaces _C.rasterize_gaussians_backward [U]. */
+int dmgs_backward(const dmgs_settings* s, int64_t P, int64_t K, const float* means3D, const float* shs,
+                  const float* colors_precomp, const float* scales, const float* rotations,
+                  const float* cov3D_precomp, const int32_t* radii, const void* geom_ws, const void* binning_ws,
+                  const void* image_ws, void* bwd_ws, const float* dL_dout, float* dL_dmeans2D,
+                  float* dL_dcolors, float* dL_dopacity, float* dL_dconic, float* dL_dmeans3D, float* dL_dcov3D,
+                  float* dL_dsh, float* dL_dscales, float* dL_drotations, dmgs_stream_t stream);
+
+/* ---- utilities -------------------------------------------------------------- */
+/* present[i] = 1 if Gaussian i lies in front of the near plane (z_view > 0.2).
+ * Replaces _C.mark_visible [U] (GaussianRasterizer.markVisible). */
+int dmgs_mark_visible(int64_t P, const float* means3D, const float* viewmatrix, uint8_t* present,
+                      dmgs_stream_t stream);
+
+/* Stable LSD radix sort of (key, value) uint32 pairs on bits [0, bits).
+ * (k0, v0) hold the input and receive the output; (k1, v1) are ping-pong
+ * buffers of n elements; scratch holds dmgs_sort_scratch_size(n) bytes. */
+size_t dmgs_sort_scratch_size(int64_t n);
+int dmgs_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int64_t n, int32_t bits,
+                    void* scratch, dmgs_stream_t stream);
+
+/* Inclusive prefix sum of n uint32 (in == out allowed). */
+size_t dmgs_scan_scratch_size(int64_t n);
+int dmgs_inclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* scratch, dmgs_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMGS_H */
