@@ -35,7 +35,11 @@ _SIGNATURES = {
     "dmgs_backward_workspace_size": (c_size_t, [c_int64]),
     "dmgs_workspace_offsets": (c_int32, [c_int64, c_int64, c_int32, c_int32, ctypes.POINTER(c_size_t)]),
     "dmgs_forward_geometry": (c_int32, [ctypes.POINTER(DmgsSettings), c_int64] + [c_void_p] * 10 + [c_void_p]),
+    "dmgs_forward_binning": (c_int32, [ctypes.POINTER(DmgsSettings), c_int64, c_int64] + [c_void_p] * 3 + [c_void_p]),
+    "dmgs_render_forward": (c_int32, [ctypes.POINTER(DmgsSettings), c_int64, c_int64] + [c_void_p] * 4 + [c_void_p]),
     "dmgs_forward_render": (c_int32, [ctypes.POINTER(DmgsSettings), c_int64, c_int64] + [c_void_p] * 4 + [c_void_p]),
+    "dmgs_render_backward": (c_int32, [ctypes.POINTER(DmgsSettings), c_int64, c_int64] + [c_void_p] * 5 + [c_void_p]),
+    "dmgs_geometry_backward": (c_int32, [ctypes.POINTER(DmgsSettings), c_int64] + [c_void_p] * 17 + [c_void_p]),
     "dmgs_backward": (c_int32, [ctypes.POINTER(DmgsSettings), c_int64, c_int64] + [c_void_p] * 21 + [c_void_p]),
     "dmgs_mark_visible": (c_int32, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dmgs_sort_scratch_size": (c_size_t, [c_int64]),

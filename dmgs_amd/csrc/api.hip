@@ -223,12 +223,13 @@ int dmgs_forward_geometry(const dmgs_settings* s, int64_t P, const float* means3
   return check_launch("count copy", st, s->debug);
 }
 
-int dmgs_forward_render(const dmgs_settings* s, int64_t P, int64_t K, void* geom_ws, void* binning_ws,
-                        void* image_ws, float* out_color, dmgs_stream_t stream) {
+int dmgs_forward_binning(const dmgs_settings* s, int64_t P, int64_t K, const void* geom_ws, void* binning_ws,
+                         void* image_ws, dmgs_stream_t stream) {
   int rc = check_settings(s);
   if (rc) return rc;
-  if (P < 0 || K < 0 || K > (int64_t)0xffffffff) return fail(DMGS_EINVAL, "invalid P / K");
-  if (!image_ws || !out_color) return fail(DMGS_EINVAL, "image_ws and out_color are required");
+  if (P < 0 || P > (int64_t)0x7fffffff || K < 0 || K > (int64_t)0xffffffff)
+    return fail(DMGS_EINVAL, "invalid P / K");
+  if (!image_ws) return fail(DMGS_EINVAL, "image_ws is required");
   if (K > 0 && (!geom_ws || !binning_ws)) return fail(DMGS_EINVAL, "geom_ws and binning_ws are required");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   float fx, fy;
@@ -237,7 +238,6 @@ int dmgs_forward_render(const dmgs_settings* s, int64_t P, int64_t K, void* geom
   const int T = gx * gy;
   ImgLayout IL(s->image_width, s->image_height);
   uint2* ranges = at<uint2>(image_ws, IL.ranges);
-  uint32_t* point_list = nullptr;
   if (K > 0) {
     GeomLayout GL((size_t)P);
     BinLayout BL((size_t)K);
@@ -249,12 +249,26 @@ int dmgs_forward_render(const dmgs_settings* s, int64_t P, int64_t K, void* geom
     dmgs::radix_sort_pairs(ka, va, at<uint32_t>(binning_ws, BL.keys_b), at<uint32_t>(binning_ws, BL.vals_b),
                            (size_t)K, tile_bits(T), at<uint32_t>(binning_ws, BL.scratch), st);
     if ((rc = check_launch("tile sort", st, s->debug))) return rc;
-    point_list = va;
     dmgs::launch_tile_ranges(ka, (size_t)K, ranges, T, st);
   } else {
     dmgs::launch_tile_ranges(nullptr, 0, ranges, T, st);
   }
-  if ((rc = check_launch("tile ranges", st, s->debug))) return rc;
+  return check_launch("tile ranges", st, s->debug);
+}
+
+int dmgs_render_forward(const dmgs_settings* s, int64_t P, int64_t K, const void* geom_ws, const void* binning_ws,
+                        void* image_ws, float* out_color, dmgs_stream_t stream) {
+  int rc = check_settings(s);
+  if (rc) return rc;
+  if (P < 0 || P > (int64_t)0x7fffffff || K < 0 || K > (int64_t)0xffffffff)
+    return fail(DMGS_EINVAL, "invalid P / K");
+  if (!image_ws || !out_color) return fail(DMGS_EINVAL, "image_ws and out_color are required");
+  if (K > 0 && (!geom_ws || !binning_ws)) return fail(DMGS_EINVAL, "geom_ws and binning_ws are required");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  float fx, fy;
+  int gx, gy;
+  fill_geom_common(s, fx, fy, gx, gy);
+  ImgLayout IL(s->image_width, s->image_height);
   dmgs::RenderArgs r{};
   r.W = s->image_width;
   r.H = s->image_height;
@@ -262,8 +276,8 @@ int dmgs_forward_render(const dmgs_settings* s, int64_t P, int64_t K, void* geom
   r.grid_y = gy;
   r.P = (uint32_t)P;
   r.K = (uint32_t)K;
-  r.ranges = ranges;
-  r.point_list = point_list;
+  r.ranges = at<uint2>(image_ws, IL.ranges);
+  r.point_list = K > 0 ? at<uint32_t>(binning_ws, BinLayout((size_t)K).vals_a) : nullptr;
   r.splats = K > 0 ? at<float4>(geom_ws, GeomLayout((size_t)P).splats) : nullptr;
   r.bg = s->bg;
   r.out_color = out_color;
@@ -273,55 +287,82 @@ int dmgs_forward_render(const dmgs_settings* s, int64_t P, int64_t K, void* geom
   return check_launch("render forward", st, s->debug);
 }
 
-int dmgs_backward(const dmgs_settings* s, int64_t P, int64_t K, const float* means3D, const float* shs,
-                  const float* colors_precomp, const float* scales, const float* rotations,
-                  const float* cov3D_precomp, const int32_t* radii, const void* geom_ws, const void* binning_ws,
-                  const void* image_ws, void* bwd_ws, const float* dL_dout, float* dL_dmeans2D,
-                  float* dL_dcolors, float* dL_dopacity, float* dL_dconic, float* dL_dmeans3D, float* dL_dcov3D,
-                  float* dL_dsh, float* dL_dscales, float* dL_drotations, dmgs_stream_t stream) {
+int dmgs_forward_render(const dmgs_settings* s, int64_t P, int64_t K, void* geom_ws, void* binning_ws,
+                        void* image_ws, float* out_color, dmgs_stream_t stream) {
+  int rc = dmgs_forward_binning(s, P, K, geom_ws, binning_ws, image_ws, stream);
+  if (rc) return rc;
+  return dmgs_render_forward(s, P, K, geom_ws, binning_ws, image_ws, out_color, stream);
+}
+
+namespace {
+int check_bwd_common(const dmgs_settings* s, int64_t P, int64_t K, const float* shs, const float* colors_precomp,
+                     const float* scales, const float* rotations, const float* cov3D_precomp) {
   int rc = check_settings(s);
   if (rc) return rc;
   if (P < 0 || P > (int64_t)0x7fffffff) return fail(DMGS_EINVAL, "invalid Gaussian count %lld", (long long)P);
+  if (K < 0 || K > (int64_t)0xffffffff) return fail(DMGS_EINVAL, "invalid K");
   if (P == 0) return DMGS_OK;
-  if (!means3D) return fail(DMGS_EINVAL, "means3D is required");
   if ((shs == nullptr) == (colors_precomp == nullptr))
     return fail(DMGS_EINVAL, "Please provide exactly one of either SHs or precomputed colors!");
   if ((scales == nullptr) != (rotations == nullptr) || (scales == nullptr) == (cov3D_precomp == nullptr))
     return fail(DMGS_EINVAL, "Please provide exactly one of either scale/rotation pair or precomputed 3D covariance!");
-  if (K < 0 || K > (int64_t)0xffffffff) return fail(DMGS_EINVAL, "invalid K");
-  if (!radii || !geom_ws || !image_ws || !bwd_ws || !dL_dout) return fail(DMGS_EINVAL, "missing buffers");
+  return DMGS_OK;
+}
+}  // namespace
+
+int dmgs_render_backward(const dmgs_settings* s, int64_t P, int64_t K, const void* geom_ws, const void* binning_ws,
+                         const void* image_ws, void* bwd_ws, const float* dL_dout, dmgs_stream_t stream) {
+  int rc = check_settings(s);
+  if (rc) return rc;
+  if (P < 0 || P > (int64_t)0x7fffffff || K < 0 || K > (int64_t)0xffffffff)
+    return fail(DMGS_EINVAL, "invalid P / K");
+  if (P == 0) return DMGS_OK;
+  if (!geom_ws || !image_ws || !bwd_ws || !dL_dout) return fail(DMGS_EINVAL, "missing buffers");
   if (K > 0 && !binning_ws) return fail(DMGS_EINVAL, "binning_ws is required");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  float fx, fy;
+  int gx, gy;
+  fill_geom_common(s, fx, fy, gx, gy);
+  ImgLayout IL(s->image_width, s->image_height);
+  float* gacc = static_cast<float*>(bwd_ws);
+  if (hipMemsetAsync(gacc, 0, (size_t)P * DMGS_GACC_STRIDE * 4, st) != hipSuccess)
+    return fail(DMGS_EHIP, "memset of the gradient accumulator failed");
+  if (K == 0) return check_launch("gradient accumulator clear", st, s->debug);
+  dmgs::RenderArgs r{};
+  r.W = s->image_width;
+  r.H = s->image_height;
+  r.grid_x = gx;
+  r.grid_y = gy;
+  r.P = (uint32_t)P;
+  r.K = (uint32_t)K;
+  r.ranges = at<uint2>(image_ws, IL.ranges);
+  r.point_list = at<uint32_t>(binning_ws, BinLayout((size_t)K).vals_a);
+  r.splats = at<float4>(geom_ws, GeomLayout((size_t)P).splats);
+  r.bg = s->bg;
+  r.final_T = const_cast<float*>(at<float>(image_ws, IL.final_T));
+  r.n_contrib = const_cast<uint32_t*>(at<uint32_t>(image_ws, IL.n_contrib));
+  r.dL_dpix = dL_dout;
+  r.gacc = gacc;
+  dmgs::launch_render_bwd(r, st);
+  return check_launch("render backward", st, s->debug);
+}
+
+int dmgs_geometry_backward(const dmgs_settings* s, int64_t P, const float* means3D, const float* shs,
+                           const float* colors_precomp, const float* scales, const float* rotations,
+                           const float* cov3D_precomp, const int32_t* radii, const void* bwd_ws, float* dL_dmeans2D,
+                           float* dL_dcolors, float* dL_dopacity, float* dL_dconic, float* dL_dmeans3D,
+                           float* dL_dcov3D, float* dL_dsh, float* dL_dscales, float* dL_drotations,
+                           dmgs_stream_t stream) {
+  int rc = check_bwd_common(s, P, 0, shs, colors_precomp, scales, rotations, cov3D_precomp);
+  if (rc) return rc;
+  if (P == 0) return DMGS_OK;
+  if (!means3D || !radii || !bwd_ws) return fail(DMGS_EINVAL, "missing buffers");
   if (shs && !dL_dsh) return fail(DMGS_EINVAL, "dL_dsh is required with shs");
   if (scales && (!dL_dscales || !dL_drotations)) return fail(DMGS_EINVAL, "dL_dscales/dL_drotations required");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   float fx, fy;
   int gx, gy;
   fill_geom_common(s, fx, fy, gx, gy);
-  GeomLayout GL((size_t)P);
-  ImgLayout IL(s->image_width, s->image_height);
-  float* gacc = static_cast<float*>(bwd_ws);
-  if (hipMemsetAsync(gacc, 0, (size_t)P * DMGS_GACC_STRIDE * 4, st) != hipSuccess)
-    return fail(DMGS_EHIP, "memset of the gradient accumulator failed");
-  if (K > 0) {
-    BinLayout BL((size_t)K);
-    dmgs::RenderArgs r{};
-    r.W = s->image_width;
-    r.H = s->image_height;
-    r.grid_x = gx;
-    r.grid_y = gy;
-    r.P = (uint32_t)P;
-    r.K = (uint32_t)K;
-    r.ranges = at<uint2>(image_ws, IL.ranges);
-    r.point_list = at<uint32_t>(binning_ws, BL.vals_a);
-    r.splats = at<float4>(geom_ws, GL.splats);
-    r.bg = s->bg;
-    r.final_T = const_cast<float*>(at<float>(image_ws, IL.final_T));
-    r.n_contrib = const_cast<uint32_t*>(at<uint32_t>(image_ws, IL.n_contrib));
-    r.dL_dpix = dL_dout;
-    r.gacc = gacc;
-    dmgs::launch_render_bwd(r, st);
-    if ((rc = check_launch("render backward", st, s->debug))) return rc;
-  }
   dmgs::GeometryBwdArgs g{};
   g.P = (int)P;
   g.W = s->image_width;
@@ -342,7 +383,7 @@ int dmgs_backward(const dmgs_settings* s, int64_t P, int64_t K, const float* mea
   g.projmatrix = s->projmatrix;
   g.campos = s->campos;
   g.radii = radii;
-  g.gacc = gacc;
+  g.gacc = static_cast<const float*>(bwd_ws);
   g.dL_dmeans2D = dL_dmeans2D;
   g.dL_dcolors = dL_dcolors;
   g.dL_dopacity = dL_dopacity;
@@ -354,6 +395,25 @@ int dmgs_backward(const dmgs_settings* s, int64_t P, int64_t K, const float* mea
   g.dL_drotations = dL_drotations;
   dmgs::launch_geometry_bwd(g, st);
   return check_launch("geometry backward", st, s->debug);
+}
+
+int dmgs_backward(const dmgs_settings* s, int64_t P, int64_t K, const float* means3D, const float* shs,
+                  const float* colors_precomp, const float* scales, const float* rotations,
+                  const float* cov3D_precomp, const int32_t* radii, const void* geom_ws, const void* binning_ws,
+                  const void* image_ws, void* bwd_ws, const float* dL_dout, float* dL_dmeans2D,
+                  float* dL_dcolors, float* dL_dopacity, float* dL_dconic, float* dL_dmeans3D, float* dL_dcov3D,
+                  float* dL_dsh, float* dL_dscales, float* dL_drotations, dmgs_stream_t stream) {
+  int rc = check_bwd_common(s, P, K, shs, colors_precomp, scales, rotations, cov3D_precomp);
+  if (rc) return rc;
+  if (P == 0) return DMGS_OK;
+  if (!means3D || !radii) return fail(DMGS_EINVAL, "missing buffers");
+  if (shs && !dL_dsh) return fail(DMGS_EINVAL, "dL_dsh is required with shs");
+  if (scales && (!dL_dscales || !dL_drotations)) return fail(DMGS_EINVAL, "dL_dscales/dL_drotations required");
+  rc = dmgs_render_backward(s, P, K, geom_ws, binning_ws, image_ws, bwd_ws, dL_dout, stream);
+  if (rc) return rc;
+  return dmgs_geometry_backward(s, P, means3D, shs, colors_precomp, scales, rotations, cov3D_precomp, radii, bwd_ws,
+                                dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dconic, dL_dmeans3D, dL_dcov3D, dL_dsh,
+                                dL_dscales, dL_drotations, stream);
 }
 
 int dmgs_mark_visible(int64_t P, const float* means3D, const float* viewmatrix, uint8_t* present,

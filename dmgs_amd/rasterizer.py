@@ -82,6 +82,30 @@ class _Frame:
         self.c = s
 
 
+class _NullStage:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+_NULL_STAGE = _NullStage()
+_stage_hook = None
+
+
+def set_stage_hook(hook):
+    """Install ``hook(name)`` returning a context manager entered around every
+    C-ABI stage call on the launching stream (used by the benchmark to time
+    stages with device events).  ``None`` removes it."""
+    global _stage_hook
+    _stage_hook = hook
+
+
+def _stage(name):
+    return _NULL_STAGE if _stage_hook is None else _stage_hook(name)
+
+
 def _cpu_copy(args):
     return tuple(a.detach().cpu() if isinstance(a, torch.Tensor) else a for a in args)
 
@@ -125,16 +149,21 @@ def forward_raw(means3D, opacities, sh, colors_precomp, scales, rotations, cov3D
     stream = N.stream_handle(dev)
     geom = torch.empty(lib.dmgs_geometry_workspace_size(P), dtype=torch.uint8, device=dev)
     count = torch.zeros(1, dtype=torch.int32, device=dev)
-    N.check(lib.dmgs_forward_geometry(ctypes_ref(fr.c), P, N.ptr(t["means3D"]), N.ptr(t["opacities"]),
-                                      N.ptr(t["shs"]), N.ptr(t["colors"]), N.ptr(t["scales"]),
-                                      N.ptr(t["rotations"]), N.ptr(t["cov3D"]), N.ptr(radii), N.ptr(geom),
-                                      N.ptr(count), stream), "dmgs_forward_geometry")
+    with _stage("geometry"):
+        N.check(lib.dmgs_forward_geometry(ctypes_ref(fr.c), P, N.ptr(t["means3D"]), N.ptr(t["opacities"]),
+                                          N.ptr(t["shs"]), N.ptr(t["colors"]), N.ptr(t["scales"]),
+                                          N.ptr(t["rotations"]), N.ptr(t["cov3D"]), N.ptr(radii), N.ptr(geom),
+                                          N.ptr(count), stream), "dmgs_forward_geometry")
     K = int(count.item())  # the one host synchronisation of the plain path (as in the reference)
     binning = torch.empty(max(1, lib.dmgs_binning_workspace_size(K)), dtype=torch.uint8, device=dev)
     image = torch.empty(lib.dmgs_image_workspace_size(W, H), dtype=torch.uint8, device=dev)
     color = torch.empty(3, H, W, dtype=torch.float32, device=dev)
-    N.check(lib.dmgs_forward_render(ctypes_ref(fr.c), P, K, N.ptr(geom), N.ptr(binning), N.ptr(image),
-                                    N.ptr(color), stream), "dmgs_forward_render")
+    with _stage("binning"):
+        N.check(lib.dmgs_forward_binning(ctypes_ref(fr.c), P, K, N.ptr(geom), N.ptr(binning), N.ptr(image),
+                                         stream), "dmgs_forward_binning")
+    with _stage("render_forward"):
+        N.check(lib.dmgs_render_forward(ctypes_ref(fr.c), P, K, N.ptr(geom), N.ptr(binning), N.ptr(image),
+                                        N.ptr(color), stream), "dmgs_render_forward")
     state = dict(frame=fr, tensors=t, geom=geom, binning=binning, image=image, K=K, P=P, M=M)
     return color, radii, state
 
@@ -152,13 +181,18 @@ def backward_raw(state, radii, grad_color):
                rotations=f(P, 4) if t["rotations"] is not None else None)
     bws = torch.empty(lib.dmgs_backward_workspace_size(P), dtype=torch.uint8, device=dev)
     g = grad_color.to(dtype=torch.float32).contiguous()
-    N.check(lib.dmgs_backward(ctypes_ref(fr.c), P, K, N.ptr(t["means3D"]), N.ptr(t["shs"]), N.ptr(t["colors"]),
-                              N.ptr(t["scales"]), N.ptr(t["rotations"]), N.ptr(t["cov3D"]), N.ptr(radii),
-                              N.ptr(state["geom"]), N.ptr(state["binning"]), N.ptr(state["image"]), N.ptr(bws),
-                              N.ptr(g), N.ptr(out["means2D"]), N.ptr(out["colors"]), N.ptr(out["opacity"]),
-                              N.ptr(out["conic"]), N.ptr(out["means3D"]), N.ptr(out["cov3D"]), N.ptr(out["sh"]),
-                              N.ptr(out["scales"]), N.ptr(out["rotations"]), N.stream_handle(dev)),
-            "dmgs_backward")
+    stream = N.stream_handle(dev)
+    with _stage("render_backward"):
+        N.check(lib.dmgs_render_backward(ctypes_ref(fr.c), P, K, N.ptr(state["geom"]), N.ptr(state["binning"]),
+                                         N.ptr(state["image"]), N.ptr(bws), N.ptr(g), stream), "dmgs_render_backward")
+    with _stage("geometry_backward"):
+        N.check(lib.dmgs_geometry_backward(ctypes_ref(fr.c), P, N.ptr(t["means3D"]), N.ptr(t["shs"]),
+                                           N.ptr(t["colors"]), N.ptr(t["scales"]), N.ptr(t["rotations"]),
+                                           N.ptr(t["cov3D"]), N.ptr(radii), N.ptr(bws), N.ptr(out["means2D"]),
+                                           N.ptr(out["colors"]), N.ptr(out["opacity"]), N.ptr(out["conic"]),
+                                           N.ptr(out["means3D"]), N.ptr(out["cov3D"]), N.ptr(out["sh"]),
+                                           N.ptr(out["scales"]), N.ptr(out["rotations"]), stream),
+                "dmgs_geometry_backward")
     return out
 
 

@@ -89,16 +89,36 @@ int dmgs_forward_geometry(const dmgs_settings* s, int64_t P, const float* means3
                           const float* rotations, const float* cov3D_precomp, int32_t* radii, void* geom_ws,
                           uint32_t* num_rendered_dev, dmgs_stream_t stream);
 
-/* Duplicate into (tile, Gaussian) instances, sort by tile, find per-tile
- * ranges and blend front to back.  K must be the count written by
- * dmgs_forward_geometry.  out_color: (3, H, W) float32.
+/* Duplicate into (tile, Gaussian) instances in depth order, stably sort by
+ * tile and find the per-tile ranges.  K must be the count written by
+ * dmgs_forward_geometry.  (duplicateWithKeys + SortPairs + identifyTileRanges [U]) */
+int dmgs_forward_binning(const dmgs_settings* s, int64_t P, int64_t K, const void* geom_ws, void* binning_ws,
+                         void* image_ws, dmgs_stream_t stream);
+/* Front-to-back blend of every tile.  out_color: (3, H, W) float32.
+ * (renderCUDA forward [U]) */
+int dmgs_render_forward(const dmgs_settings* s, int64_t P, int64_t K, const void* geom_ws, const void* binning_ws,
+                        void* image_ws, float* out_color, dmgs_stream_t stream);
+/* dmgs_forward_binning followed by dmgs_render_forward.
  * Replaces the second half of _C.rasterize_gaussians [U]. */
 int dmgs_forward_render(const dmgs_settings* s, int64_t P, int64_t K, void* geom_ws, void* binning_ws,
                         void* image_ws, float* out_color, dmgs_stream_t stream);
 
 /* ---- backward ------------------------------------------------------------- */
-/* Reverse-traversal blend backward + fused cov2D / projection / SH / cov3D
- * backward.  Inputs are the forward's inputs and workspaces; dL_dout is
+/* Blend backward (renderCUDA backward [U]): clears the accumulator in bwd_ws
+ * and adds per-Gaussian dL/d{mean2D, conic, opacity, colour} into it. */
+int dmgs_render_backward(const dmgs_settings* s, int64_t P, int64_t K, const void* geom_ws, const void* binning_ws,
+                         const void* image_ws, void* bwd_ws, const float* dL_dout, dmgs_stream_t stream);
+/* Per-Gaussian backward from the accumulator (computeCov2DCUDA + preprocessCUDA
+ * backward [U], fused); outputs as for dmgs_backward. */
+int dmgs_geometry_backward(const dmgs_settings* s, int64_t P, const float* means3D, const float* shs,
+                           const float* colors_precomp, const float* scales, const float* rotations,
+                           const float* cov3D_precomp, const int32_t* radii, const void* bwd_ws, float* dL_dmeans2D,
+                           float* dL_dcolors, float* dL_dopacity, float* dL_dconic, float* dL_dmeans3D,
+                           float* dL_dcov3D, float* dL_dsh, float* dL_dscales, float* dL_drotations,
+                           dmgs_stream_t stream);
+/* dmgs_render_backward followed by dmgs_geometry_backward: reverse-traversal
+ * blend backward + fused cov2D / projection / SH / cov3D backward.
+ * Inputs are the forward's inputs and workspaces; dL_dout is
  * (3, H, W).  Outputs (caller-allocated, fully overwritten):
  *   dL_dmeans2D (P,3) NDC units, z = 0; dL_dcolors (P,3); dL_dopacity (P,1);
  *   dL_dconic (P,4) (.z = 0); dL_dmeans3D (P,3); dL_dcov3D (P,6);

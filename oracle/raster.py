@@ -336,9 +336,17 @@ def _tile_walk(valid, alpha):
     return include, Tbef, Tfin, last, sidx, Tinc
 
 
-def render_forward(ranges, vals, xy, conic, opacity, colors, bg, W, H, return_aux=False):
+def _tiles_to_visit(ranges, tiles):
+    nonempty = np.nonzero(ranges[:, 1] > ranges[:, 0])[0]
+    if tiles is None:
+        return nonempty
+    return np.intersect1d(nonempty, np.asarray(tiles))
+
+
+def render_forward(ranges, vals, xy, conic, opacity, colors, bg, W, H, tiles=None):
     """renderCUDA forward [U] in FP64 (inputs may be FP32 device outputs).
-    Returns (out_color (C,H,W), final_T (H,W), n_contrib (H,W), ambiguous (H,W))."""
+    Returns (out_color (C,H,W), final_T (H,W), n_contrib (H,W), ambiguous (H,W)).
+    ``tiles`` restricts the work to a subset of tile ids (for timing samples)."""
     xy, conic, opac, col = _d(xy), _d(conic), _d(opacity).reshape(-1), _d(colors)
     bg = _d(bg).reshape(-1)
     C = col.shape[1]
@@ -347,7 +355,7 @@ def render_forward(ranges, vals, xy, conic, opacity, colors, bg, W, H, return_au
     n_contrib = torch.zeros(H, W, dtype=torch.int64)
     ambig = torch.zeros(H, W, dtype=torch.bool)
     vals_t = torch.as_tensor(np.asarray(vals), dtype=torch.int64)
-    for tile in np.nonzero(ranges[:, 1] > ranges[:, 0])[0]:
+    for tile in _tiles_to_visit(ranges, tiles):
         s, e = int(ranges[tile, 0]), int(ranges[tile, 1])
         ids = vals_t[s:e]
         px, py = _tile_pixels(int(tile), W, H)
@@ -368,7 +376,7 @@ def render_forward(ranges, vals, xy, conic, opacity, colors, bg, W, H, return_au
     return out, final_T, n_contrib, ambig
 
 
-def render_backward(ranges, vals, xy, conic, opacity, colors, bg, W, H, dL_dpix):
+def render_backward(ranges, vals, xy, conic, opacity, colors, bg, W, H, dL_dpix, tiles=None):
     """renderCUDA backward [U] in FP64, restated in suffix-sum form instead of the
     device's reverse traversal.  Returns per-Gaussian (dL_dmean2D (P,3) NDC units,
     dL_dconic (P,4) with the reference's half-valued .y, dL_dopacity (P,1),
@@ -382,7 +390,7 @@ def render_backward(ranges, vals, xy, conic, opacity, colors, bg, W, H, dL_dpix)
     dopac = torch.zeros(P, dtype=F64)
     dcol = torch.zeros(P, C, dtype=F64)
     vals_t = torch.as_tensor(np.asarray(vals), dtype=torch.int64)
-    for tile in np.nonzero(ranges[:, 1] > ranges[:, 0])[0]:
+    for tile in _tiles_to_visit(ranges, tiles):
         s, e = int(ranges[tile, 0]), int(ranges[tile, 1])
         ids = vals_t[s:e]
         px, py = _tile_pixels(int(tile), W, H)
