@@ -6,8 +6,9 @@ Workload (default ``--workload single``): 1M free Gaussians in FP32 — means a
 N(-3.5, 0.4), normalised N(0,1)^4 quaternions, sigmoid(N(0,1)) opacity, SH
 degree 3 (DC N(0, 0.5), rest N(0, 0.1)) — seen by one 800x800 pinhole camera
 (FoV 50 deg, distance 4, white background).  One step = drop-in ``render`` +
-0.8 L1 + 0.2 (1 - SSIM) against a seeded target + ``backward`` into all six
-parameter tensors.  Inputs are resident in HBM before timing starts.
+0.8 L1 + 0.2 (1 - SSIM) against a seeded target (fused HIP loss kernels, same
+objective as utils/loss_utils.py) + ``backward`` into all six parameter
+tensors.  Inputs are resident in HBM before timing starts.
 
 Prints ONE JSON line.  ``roofline`` is for the dominant kernel, timed with
 device events on the launching stream over the timed region; ``cpu_baseline``
@@ -160,7 +161,7 @@ def main():
     from dmgs_amd import cameras, scenes
     from dmgs_amd import rasterizer as RZ
     from dmgs_amd.gaussian_renderer import PipelineParams, render
-    from dmgs_amd.loss import dssim_l1
+    from dmgs_amd.loss import fused_dssim_l1
 
     dev = "cuda"
     torch.cuda.set_device(0)
@@ -175,7 +176,7 @@ def main():
         for p in model.parameters():
             p.grad = None
         out = render(cam, model, pipe, bg)
-        loss = dssim_l1(out["render"], target, 0.2)
+        loss = fused_dssim_l1(out["render"], target, 0.2)
         loss.backward()
         return out
 

@@ -44,6 +44,11 @@ _SIGNATURES = {
     "dmgs_mark_visible": (c_int32, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dmgs_sort_scratch_size": (c_size_t, [c_int64]),
     "dmgs_sort_pairs": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    "dmgs_dssim_l1_workspace_size": (c_size_t, [c_int32, c_int32, c_int32]),
+    "dmgs_dssim_l1_forward": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_float, c_void_p, c_void_p,
+                                        c_void_p]),
+    "dmgs_dssim_l1_backward": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_float, c_void_p,
+                                         c_void_p, c_void_p, c_void_p]),
     "dmgs_scan_scratch_size": (c_size_t, [c_int64]),
     "dmgs_inclusive_scan_u32": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
 }

@@ -64,7 +64,7 @@ struct BinLayout {
     keys_b = o; o += align_up(K * 4);
     vals_a = o; o += align_up(K * 4);
     vals_b = o; o += align_up(K * 4);
-    scratch = o; o += align_up(dmgs::radix_scratch_words(K) * 4);
+    scratch = o; o += align_up(std::max(dmgs::radix_scratch_words(K), dmgs::duplicate_scratch_words(K)) * 4);
     total = o;
   }
 };
@@ -244,7 +244,8 @@ int dmgs_forward_binning(const dmgs_settings* s, int64_t P, int64_t K, const voi
     uint32_t* ka = at<uint32_t>(binning_ws, BL.keys_a);
     uint32_t* va = at<uint32_t>(binning_ws, BL.vals_a);
     dmgs::launch_duplicate((int)P, at<uint32_t>(geom_ws, GL.order_a), at<uint32_t>(geom_ws, GL.offsets),
-                           at<float4>(geom_ws, GL.splats), gx, gy, ka, va, (uint32_t)K, st);
+                           at<float4>(geom_ws, GL.splats), gx, gy, ka, va, (uint32_t)K,
+                           at<uint32_t>(binning_ws, BL.scratch), st);
     if ((rc = check_launch("duplicate", st, s->debug))) return rc;
     dmgs::radix_sort_pairs(ka, va, at<uint32_t>(binning_ws, BL.keys_b), at<uint32_t>(binning_ws, BL.vals_b),
                            (size_t)K, tile_bits(T), at<uint32_t>(binning_ws, BL.scratch), st);
@@ -448,6 +449,69 @@ int dmgs_inclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* 
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   dmgs::launch_inclusive_scan(in, out, (size_t)n, static_cast<uint32_t*>(scratch), st);
   return check_launch("scan", st, false);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// fused L1 + SSIM loss
+// ---------------------------------------------------------------------------
+namespace {
+void ssim_window(float w[11]) {
+  // the reference builds the window in float32: exp values, then divide by their sum
+  float sum = 0.f;
+  for (int x = 0; x < 11; ++x) {
+    w[x] = (float)std::exp(-(double)((x - 5) * (x - 5)) / (2.0 * 1.5 * 1.5));
+    sum += w[x];
+  }
+  for (int x = 0; x < 11; ++x) w[x] = w[x] / sum;
+}
+
+struct SsimLayout {
+  size_t coef, partials, out, total;
+  SsimLayout(int C, int H, int W) {
+    size_t o = 0;
+    coef = o; o += align_up((size_t)3 * C * H * W * 4);
+    partials = o; o += align_up(dmgs::dssim_partials(C, H, W) * 4);
+    out = o; o += align_up(4 * 4);
+    total = o;
+  }
+};
+}  // namespace
+
+extern "C" {
+
+size_t dmgs_dssim_l1_workspace_size(int32_t C, int32_t H, int32_t W) {
+  return (C <= 0 || H <= 0 || W <= 0) ? 0 : SsimLayout(C, H, W).total;
+}
+
+int dmgs_dssim_l1_forward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
+                          void* ws, float* loss_out, dmgs_stream_t stream) {
+  if (!img || !gt || !ws || !loss_out) return fail(DMGS_EINVAL, "null pointer");
+  if (C <= 0 || H <= 0 || W <= 0 || (int64_t)C * H * W > ((int64_t)1 << 31))
+    return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d", C, H, W);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  SsimLayout L(C, H, W);
+  float w[11];
+  ssim_window(w);
+  dmgs::launch_dssim_forward(img, gt, C, H, W, w, lambda_dssim, at<float>(ws, L.coef), at<float>(ws, L.partials),
+                             at<float>(ws, L.out), st);
+  if (hipMemcpyAsync(loss_out, at<float>(ws, L.out), 3 * sizeof(float), hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return fail(DMGS_EHIP, "loss copy failed");
+  return check_launch("dssim forward", st, false);
+}
+
+int dmgs_dssim_l1_backward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
+                           const void* ws, const float* dloss, float* dimg, dmgs_stream_t stream) {
+  if (!img || !gt || !ws || !dloss || !dimg) return fail(DMGS_EINVAL, "null pointer");
+  if (C <= 0 || H <= 0 || W <= 0 || (int64_t)C * H * W > ((int64_t)1 << 31))
+    return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d", C, H, W);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  SsimLayout L(C, H, W);
+  float w[11];
+  ssim_window(w);
+  dmgs::launch_dssim_backward(img, gt, at<float>(ws, L.coef), C, H, W, w, lambda_dssim, dloss, dimg, st);
+  return check_launch("dssim backward", st, false);
 }
 
 }  // extern "C"

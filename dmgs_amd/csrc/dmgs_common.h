@@ -100,4 +100,23 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   return v;
 }
 
+// Load the first 16 SH coefficients (RGB each) of one Gaussian into registers;
+// 16-byte vector loads when the per-Gaussian stride allows it.
+__device__ __forceinline__ void load_sh16(const float* __restrict__ sh, int M, float (&r)[48]) {
+  if (M == 16 && ((reinterpret_cast<uintptr_t>(sh) & 15) == 0)) {
+    const float4* s4 = reinterpret_cast<const float4*>(sh);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+      const float4 v = s4[q];
+      r[4 * q] = v.x;
+      r[4 * q + 1] = v.y;
+      r[4 * q + 2] = v.z;
+      r[4 * q + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 48; ++i) r[i] = (i / 3 < M) ? sh[i] : 0.f;
+  }
+}
+
 }  // namespace dmgs

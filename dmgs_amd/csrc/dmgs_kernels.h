@@ -50,8 +50,11 @@ void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, 
 
 // Emit (tile, gaussian) pairs in depth order.
 // The splat record carries the integer radius (0 = culled) in s2.z.
+// scratch holds duplicate_scratch_words(K) words.
+size_t duplicate_scratch_words(size_t K);
 void launch_duplicate(int P, const uint32_t* order, const uint32_t* offsets, const float4* splats, int grid_x,
-                      int grid_y, uint32_t* tile_keys, uint32_t* values, uint32_t K, hipStream_t s);
+                      int grid_y, uint32_t* tile_keys, uint32_t* values, uint32_t K, uint32_t* scratch,
+                      hipStream_t s);
 void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s);
 
 // ---- blend ----
@@ -100,5 +103,12 @@ struct GeometryBwdArgs {
   float* dL_drotations;  // (P, 4)
 };
 void launch_geometry_bwd(const GeometryBwdArgs& a, hipStream_t s);
+
+// ---- fused L1 + SSIM loss ----
+size_t dssim_partials(int C, int H, int W);
+void launch_dssim_forward(const float* x, const float* y, int C, int H, int W, const float* w11, float lam,
+                          float* coef, float* partials, float* out, hipStream_t s);
+void launch_dssim_backward(const float* x, const float* y, const float* coef, int C, int H, int W, const float* w11,
+                           float lam, const float* dloss, float* dx, hipStream_t s);
 
 }  // namespace dmgs

@@ -27,72 +27,78 @@ __constant__ float kB_C3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.4570
                                -0.5900435899266435f};
 
 // SH backward: recomputes the colour (for the clamp mask), writes dL/dsh and
-// returns dL/d(mean) through the normalised view direction.
-__device__ __forceinline__ float3 sh_backward(int deg, int M, const float* __restrict__ sh, float3 pos,
+// returns dL/d(mean) through the normalised view direction.  The basis
+// partials along x, y, z are those of the polynomials in sh_to_rgb.
+__device__ __forceinline__ float3 sh_backward(int deg, int M, const float* __restrict__ shp, float3 pos,
                                               const float* campos, float3 dL_dRGB, float* __restrict__ dsh) {
   const float3 v = make_float3(pos.x - campos[0], pos.y - campos[1], pos.z - campos[2]);
   const float inv_len = 1.0f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
   const float x = v.x * inv_len, y = v.y * inv_len, z = v.z * inv_len;
-
-  // basis values b[k] and their partial derivatives along x, y, z
-  float b[16], bx[16], by[16], bz[16];
+  float sh[48];
+  load_sh16(shp, M, sh);
   const int n = (deg + 1) * (deg + 1);
+
+  float b[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) b[k] = bx[k] = by[k] = bz[k] = 0.f;
-  b[0] = kB_C0;
+  for (int k = 0; k < 16; ++k) b[k] = 0.f;
+  float3 rgb = make_float3(0.5f, 0.5f, 0.5f);
+  float3 dRx = make_float3(0.f, 0.f, 0.f), dRy = dRx, dRz = dRx;
+  // accumulate colour and the three directional partials for coefficient k
+  auto acc = [&](int k, float bk, float bxk, float byk, float bzk) {
+    const float s0 = sh[3 * k], s1 = sh[3 * k + 1], s2 = sh[3 * k + 2];
+    b[k] = bk;
+    rgb.x += bk * s0; rgb.y += bk * s1; rgb.z += bk * s2;
+    dRx.x += bxk * s0; dRx.y += bxk * s1; dRx.z += bxk * s2;
+    dRy.x += byk * s0; dRy.y += byk * s1; dRy.z += byk * s2;
+    dRz.x += bzk * s0; dRz.y += bzk * s1; dRz.z += bzk * s2;
+  };
+  acc(0, kB_C0, 0.f, 0.f, 0.f);
   if (deg > 0) {
-    b[1] = -kB_C1 * y; by[1] = -kB_C1;
-    b[2] = kB_C1 * z;  bz[2] = kB_C1;
-    b[3] = -kB_C1 * x; bx[3] = -kB_C1;
+    acc(1, -kB_C1 * y, 0.f, -kB_C1, 0.f);
+    acc(2, kB_C1 * z, 0.f, 0.f, kB_C1);
+    acc(3, -kB_C1 * x, -kB_C1, 0.f, 0.f);
     if (deg > 1) {
       const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-      b[4] = kB_C2[0] * xy;  bx[4] = kB_C2[0] * y; by[4] = kB_C2[0] * x;
-      b[5] = kB_C2[1] * yz;  by[5] = kB_C2[1] * z; bz[5] = kB_C2[1] * y;
-      b[6] = kB_C2[2] * (2.f * zz - xx - yy);
-      bx[6] = -2.f * kB_C2[2] * x; by[6] = -2.f * kB_C2[2] * y; bz[6] = 4.f * kB_C2[2] * z;
-      b[7] = kB_C2[3] * xz;  bx[7] = kB_C2[3] * z; bz[7] = kB_C2[3] * x;
-      b[8] = kB_C2[4] * (xx - yy); bx[8] = 2.f * kB_C2[4] * x; by[8] = -2.f * kB_C2[4] * y;
+      acc(4, kB_C2[0] * xy, kB_C2[0] * y, kB_C2[0] * x, 0.f);
+      acc(5, kB_C2[1] * yz, 0.f, kB_C2[1] * z, kB_C2[1] * y);
+      acc(6, kB_C2[2] * (2.f * zz - xx - yy), -2.f * kB_C2[2] * x, -2.f * kB_C2[2] * y, 4.f * kB_C2[2] * z);
+      acc(7, kB_C2[3] * xz, kB_C2[3] * z, 0.f, kB_C2[3] * x);
+      acc(8, kB_C2[4] * (xx - yy), 2.f * kB_C2[4] * x, -2.f * kB_C2[4] * y, 0.f);
       if (deg > 2) {
-        b[9] = kB_C3[0] * y * (3.f * xx - yy);
-        bx[9] = kB_C3[0] * 6.f * xy; by[9] = kB_C3[0] * 3.f * (xx - yy);
-        b[10] = kB_C3[1] * xy * z;
-        bx[10] = kB_C3[1] * yz; by[10] = kB_C3[1] * xz; bz[10] = kB_C3[1] * xy;
-        b[11] = kB_C3[2] * y * (4.f * zz - xx - yy);
-        bx[11] = kB_C3[2] * (-2.f * xy); by[11] = kB_C3[2] * (4.f * zz - xx - 3.f * yy);
-        bz[11] = kB_C3[2] * 8.f * yz;
-        b[12] = kB_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy);
-        bx[12] = kB_C3[3] * (-6.f * xz); by[12] = kB_C3[3] * (-6.f * yz);
-        bz[12] = kB_C3[3] * 3.f * (2.f * zz - xx - yy);
-        b[13] = kB_C3[4] * x * (4.f * zz - xx - yy);
-        bx[13] = kB_C3[4] * (4.f * zz - 3.f * xx - yy); by[13] = kB_C3[4] * (-2.f * xy);
-        bz[13] = kB_C3[4] * 8.f * xz;
-        b[14] = kB_C3[5] * z * (xx - yy);
-        bx[14] = kB_C3[5] * 2.f * xz; by[14] = kB_C3[5] * (-2.f * yz); bz[14] = kB_C3[5] * (xx - yy);
-        b[15] = kB_C3[6] * x * (xx - 3.f * yy);
-        bx[15] = kB_C3[6] * 3.f * (xx - yy); by[15] = kB_C3[6] * (-6.f * xy);
+        acc(9, kB_C3[0] * y * (3.f * xx - yy), kB_C3[0] * 6.f * xy, kB_C3[0] * 3.f * (xx - yy), 0.f);
+        acc(10, kB_C3[1] * xy * z, kB_C3[1] * yz, kB_C3[1] * xz, kB_C3[1] * xy);
+        acc(11, kB_C3[2] * y * (4.f * zz - xx - yy), kB_C3[2] * (-2.f * xy), kB_C3[2] * (4.f * zz - xx - 3.f * yy),
+            kB_C3[2] * 8.f * yz);
+        acc(12, kB_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy), kB_C3[3] * (-6.f * xz), kB_C3[3] * (-6.f * yz),
+            kB_C3[3] * 3.f * (2.f * zz - xx - yy));
+        acc(13, kB_C3[4] * x * (4.f * zz - xx - yy), kB_C3[4] * (4.f * zz - 3.f * xx - yy), kB_C3[4] * (-2.f * xy),
+            kB_C3[4] * 8.f * xz);
+        acc(14, kB_C3[5] * z * (xx - yy), kB_C3[5] * 2.f * xz, kB_C3[5] * (-2.f * yz), kB_C3[5] * (xx - yy));
+        acc(15, kB_C3[6] * x * (xx - 3.f * yy), kB_C3[6] * 3.f * (xx - yy), kB_C3[6] * (-6.f * xy), 0.f);
       }
     }
   }
-  // colour (for the clamp mask) and direction gradient
-  float3 rgb = make_float3(0.5f, 0.5f, 0.5f);
-  float3 dRx = make_float3(0.f, 0.f, 0.f), dRy = dRx, dRz = dRx;
-  for (int k = 0; k < n; ++k) {
-    const float s0 = sh[3 * k], s1 = sh[3 * k + 1], s2 = sh[3 * k + 2];
-    rgb.x += b[k] * s0; rgb.y += b[k] * s1; rgb.z += b[k] * s2;
-    dRx.x += bx[k] * s0; dRx.y += bx[k] * s1; dRx.z += bx[k] * s2;
-    dRy.x += by[k] * s0; dRy.y += by[k] * s1; dRy.z += by[k] * s2;
-    dRz.x += bz[k] * s0; dRz.y += bz[k] * s1; dRz.z += bz[k] * s2;
-  }
-  // (rgb was accumulated from 0.5 upward; the reference adds 0.5 last, the
-  //  clamp decision differs only for values within rounding of zero)
+  (void)n;
   const float gx = rgb.x < 0.f ? 0.f : dL_dRGB.x;
   const float gy = rgb.y < 0.f ? 0.f : dL_dRGB.y;
   const float gz = rgb.z < 0.f ? 0.f : dL_dRGB.z;
-  for (int k = 0; k < M; ++k) {
-    const float bk = k < n ? b[k] : 0.f;
-    dsh[3 * k] = bk * gx;
-    dsh[3 * k + 1] = bk * gy;
-    dsh[3 * k + 2] = bk * gz;
+  // dL/dsh_k = b_k * g (zero past the active degree: b_k = 0 there)
+  if (M == 16 && ((reinterpret_cast<uintptr_t>(dsh) & 15) == 0)) {
+    float4* d4 = reinterpret_cast<float4*>(dsh);
+    const float gg[3] = {gx, gy, gz};
+#pragma unroll
+    for (int q = 0; q < 12; ++q)
+      d4[q] = make_float4(b[(4 * q) / 3] * gg[(4 * q) % 3], b[(4 * q + 1) / 3] * gg[(4 * q + 1) % 3],
+                          b[(4 * q + 2) / 3] * gg[(4 * q + 2) % 3], b[(4 * q + 3) / 3] * gg[(4 * q + 3) % 3]);
+  } else {
+    for (int k = 0; k < M; ++k) {
+      float bk = 0.f;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) bk = (j == k) ? b[j] : bk;
+      dsh[3 * k] = bk * gx;
+      dsh[3 * k + 1] = bk * gy;
+      dsh[3 * k + 2] = bk * gz;
+    }
   }
   const float3 ddir = make_float3(dRx.x * gx + dRx.y * gy + dRx.z * gz, dRy.x * gx + dRy.y * gy + dRy.z * gz,
                                   dRz.x * gx + dRz.y * gy + dRz.z * gz);

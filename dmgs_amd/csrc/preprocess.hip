@@ -25,7 +25,7 @@ __constant__ float kSH_C3[7] = {-0.5900435899266435f, 2.890611442640554f, -0.457
 
 // RGB from SH coefficients laid out (M, 3) per Gaussian; returns the raw
 // (unclamped) value + 0.5.
-__device__ __forceinline__ float3 sh_to_rgb(int deg, const float* __restrict__ sh, float3 dir) {
+__device__ __forceinline__ float3 sh_to_rgb(int deg, const float (&sh)[48], float3 dir) {
   const float x = dir.x, y = dir.y, z = dir.z;
   float3 r;
   r.x = kSH_C0 * sh[0];
@@ -164,7 +164,9 @@ __global__ __launch_bounds__(256) void preprocess_fwd_kernel(PreprocessArgs a) {
     dir.x *= inv_len;
     dir.y *= inv_len;
     dir.z *= inv_len;
-    rgb = sh_to_rgb(a.sh_degree, a.shs + (size_t)idx * a.sh_coeffs * 3, dir);
+    float sh[48];
+    load_sh16(a.shs + (size_t)idx * a.sh_coeffs * 3, a.sh_coeffs, sh);
+    rgb = sh_to_rgb(a.sh_degree, sh, dir);
     rgb.x = fmaxf(rgb.x, 0.0f);
     rgb.y = fmaxf(rgb.y, 0.0f);
     rgb.z = fmaxf(rgb.z, 0.0f);

@@ -15,7 +15,8 @@
 // n_contrib of the tile (entries behind every pixel's last contributor
 // cannot contribute), transmittance is recomputed by division instead of
 // being stored, and each wave reduces its 9 per-Gaussian partial gradients
-// with DPP row sums before a single lane-parallel atomic add per wave.
+// with DPP row sums into a per-batch LDS accumulator; the workgroup flushes
+// that accumulator with one global atomic per (Gaussian, value) per batch.
 #include "dmgs_common.h"
 #include "dmgs_kernels.h"
 
@@ -106,6 +107,7 @@ __global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_fwd_kernel(RenderArgs 
 __global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_bwd_kernel(RenderArgs a) {
   __shared__ SplatLDS L;
   __shared__ uint32_t s_max[DMGS_BLOCK_SIZE / DMGS_WAVE];
+  __shared__ float s_acc[DMGS_BLOCK_SIZE][DMGS_GACC_USED];
   const int tile = blockIdx.x;
   const int tx = tile % a.grid_x, ty = tile / a.grid_x;
   const int lx = threadIdx.x & (DMGS_BLOCK_X - 1), ly = threadIdx.x / DMGS_BLOCK_X;
@@ -145,10 +147,13 @@ __global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_bwd_kernel(RenderArgs 
   float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
   float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
 
+  const unsigned row_lane = lane & 15;  // lane within its DPP row of 16
   for (int r = 0; r < rounds; ++r) {
     __syncthreads();
     const int pos = r * DMGS_BLOCK_SIZE + threadIdx.x;
     if (pos < todo_total) stage_splat(L, threadIdx.x, a.splats, a.point_list[range.x + todo_total - pos - 1], a.P);
+#pragma unroll
+    for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[threadIdx.x][k] = 0.f;
     __syncthreads();
     const int n = min(DMGS_BLOCK_SIZE, todo_total - r * DMGS_BLOCK_SIZE);
     for (int j = 0; j < n; ++j) {
@@ -195,15 +200,24 @@ __global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_bwd_kernel(RenderArgs 
         }
       }
       if (__ballot(contrib) == 0) continue;  // wave-uniform
+      // sum over each 16-lane row, then lane k of every row adds value k into
+      // the batch accumulator in LDS (one ds_add with 4 x 9 active lanes)
       float mine = 0.f;
 #pragma unroll
       for (int k = 0; k < DMGS_GACC_USED; ++k) {
-        const float s = wave_sum(v[k]);
-        mine = (lane == (unsigned)k) ? s : mine;
+        const float s = row16_sum(v[k]);
+        mine = (row_lane == (unsigned)k) ? s : mine;
       }
-      if (lane < DMGS_GACC_USED) {
-        const uint32_t gid = __float_as_uint(L.rgb_id[j].w);
-        unsafeAtomicAdd(a.gacc + (size_t)gid * DMGS_GACC_STRIDE + lane, mine);
+      if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[j][row_lane], mine);
+    }
+    __syncthreads();
+    // flush the batch: consecutive lanes cover the 9 values of one Gaussian
+    for (int i = threadIdx.x; i < n * DMGS_GACC_USED; i += DMGS_BLOCK_SIZE) {
+      const int slot = i / DMGS_GACC_USED, k = i - slot * DMGS_GACC_USED;
+      const float val = s_acc[slot][k];
+      if (val != 0.f) {
+        const uint32_t gid = __float_as_uint(L.rgb_id[slot].w);
+        unsafeAtomicAdd(a.gacc + (size_t)gid * DMGS_GACC_STRIDE + k, val);
       }
     }
   }

@@ -25,9 +25,6 @@ constexpr int kScanTile = kScanThreads * kScanItems;  // 4096
 
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096
-constexpr int kWaveSlice = 64 * kSortItems;           // 1024
 
 // ---------------------------------------------------------------------------
 // block-wide helpers
@@ -140,7 +137,14 @@ void launch_inclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t
 
 // ---------------------------------------------------------------------------
 // radix sort
+//
+// One pass = histogram (per-block digit counts, digit-major) -> per-digit
+// exclusive scan over blocks (one workgroup per digit) -> rank + scatter.
+// The scatter derives each digit's global base from the digit totals itself,
+// so a pass is three launches.  Blocks hold 256 x ITEMS keys; small inputs use
+// fewer items per thread so that every CU gets work.
 // ---------------------------------------------------------------------------
+template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, size_t n,
                                                                  int shift, int nbits,
                                                                  uint32_t* __restrict__ hist, uint32_t nb) {
@@ -148,9 +152,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
   const uint32_t mask = (1u << nbits) - 1u;
   cnt[threadIdx.x] = 0;
   __syncthreads();
-  const size_t base = (size_t)blockIdx.x * kSortTile;
+  const size_t base = (size_t)blockIdx.x * (kSortThreads * ITEMS);
 #pragma unroll 4
-  for (int i = 0; i < kSortItems; ++i) {
+  for (int i = 0; i < ITEMS; ++i) {
     const size_t idx = base + (size_t)i * kSortThreads + threadIdx.x;
     if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & mask], 1u);
   }
@@ -158,25 +162,66 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
   if (threadIdx.x <= mask) hist[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Exclusive scan of row `blockIdx.x` (one digit, nb block counts) in place;
+// the row total goes to totals[digit].
+__global__ __launch_bounds__(kScanThreads) void radix_rowscan_kernel(uint32_t* __restrict__ hist, uint32_t nb,
+                                                                    uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_wave[kScanThreads / 64];
+  uint32_t* row = hist + (size_t)blockIdx.x * nb;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += kScanTile) {
+    uint32_t v[kScanItems];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const uint32_t idx = base + threadIdx.x * kScanItems + i;
+      v[i] = idx < nb ? row[idx] : 0u;
+      tsum += v[i];
+    }
+    uint32_t total;
+    uint32_t run = carry + block_exclusive_scan(tsum, s_wave, total);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const uint32_t idx = base + threadIdx.x * kScanItems + i;
+      if (idx < nb) row[idx] = run;
+      run += v[i];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, size_t n, int shift, int nbits, const uint32_t* __restrict__ hist_excl,
-    uint32_t nb) {
+    const uint32_t* __restrict__ totals, uint32_t nb) {
+  constexpr int kTile = kSortThreads * ITEMS;
+  constexpr int kSlice = 64 * ITEMS;
   __shared__ uint32_t wcnt[kSortWaves][256];
+  __shared__ uint32_t dstart[256];  // block-local start of each digit
+  __shared__ uint32_t gbase[256];   // global position of the block's first element of each digit
+  __shared__ uint32_t skey[kTile], sval[kTile];
+  __shared__ uint32_t s_wave[kSortWaves];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t mask = (1u << nbits) - 1u;
   for (int w = 0; w < kSortWaves; ++w) wcnt[w][threadIdx.x] = 0;
   __syncthreads();
 
-  const size_t wbase = (size_t)blockIdx.x * kSortTile + (size_t)wave * kWaveSlice;
+  const size_t wbase = (size_t)blockIdx.x * kTile + (size_t)wave * kSlice;
   const uint64_t lt = lanemask_lt();
-  uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+  uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
 #pragma unroll
-  for (int i = 0; i < kSortItems; ++i) {
+  for (int i = 0; i < ITEMS; ++i) {
     const size_t idx = wbase + (size_t)i * 64 + lane;
     const bool valid = idx < n;
     key[i] = valid ? kin[idx] : 0u;
     val[i] = valid ? vin[idx] : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const size_t idx = wbase + (size_t)i * 64 + lane;
+    const bool valid = idx < n;
     const uint32_t d = (key[i] >> shift) & mask;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < nbits; ++b) {
@@ -190,42 +235,64 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     if (valid && lane == leader) wcnt[wave][d] = before + (uint32_t)__popcll(peers);
   }
   __syncthreads();
-  // exclusive prefix over waves per digit, plus this block's global offset
-  if (threadIdx.x <= mask) {
-    uint32_t run = hist_excl[(size_t)threadIdx.x * nb + blockIdx.x];
+  // per digit (one thread each): exclusive prefix over waves, block-local
+  // digit start, and the global position of the block's first element
+  const uint32_t dg = threadIdx.x;
+  uint32_t tot = 0;
+  if (dg <= mask) {
     for (int w = 0; w < kSortWaves; ++w) {
-      const uint32_t c = wcnt[w][threadIdx.x];
-      wcnt[w][threadIdx.x] = run;
-      run += c;
+      const uint32_t c = wcnt[w][dg];
+      wcnt[w][dg] = tot;
+      tot += c;
     }
   }
+  uint32_t block_total, all_total;
+  const uint32_t start = block_exclusive_scan(tot, s_wave, block_total);
+  const uint32_t digit_base = block_exclusive_scan(dg <= mask ? totals[dg] : 0u, s_wave, all_total);
+  dstart[dg] = start;
+  if (dg <= mask) gbase[dg] = digit_base + hist_excl[(size_t)dg * nb + blockIdx.x];
   __syncthreads();
+  // stage the block's pairs in LDS in sorted order
 #pragma unroll
-  for (int i = 0; i < kSortItems; ++i) {
+  for (int i = 0; i < ITEMS; ++i) {
     const size_t idx = wbase + (size_t)i * 64 + lane;
     if (idx < n) {
       const uint32_t d = (key[i] >> shift) & mask;
-      const uint32_t pos = wcnt[wave][d] + rank[i];
-      if (pos < n) {  // always true for a consistent histogram; guards the write
-        kout[pos] = key[i];
-        vout[pos] = val[i];
+      const uint32_t lpos = dstart[d] + wcnt[wave][d] + rank[i];
+      if (lpos < (uint32_t)kTile) {
+        skey[lpos] = key[i];
+        sval[lpos] = val[i];
       }
+    }
+  }
+  __syncthreads();
+  // coalesced write-out: each digit's run is contiguous in LDS and in memory
+  const size_t bbase = (size_t)blockIdx.x * kTile;
+  const uint32_t count = (uint32_t)min((size_t)kTile, n - bbase);
+  for (uint32_t i = threadIdx.x; i < count; i += kSortThreads) {
+    const uint32_t k = skey[i];
+    const uint32_t d = (k >> shift) & mask;
+    const uint32_t pos = gbase[d] + (i - dstart[d]);
+    if (pos < n) {  // always true for a consistent histogram; guards the write
+      kout[pos] = k;
+      vout[pos] = sval[i];
     }
   }
 }
 
+static int sort_items(size_t n) { return n >= ((size_t)4 << 20) ? 16 : 4; }
+
 size_t radix_scratch_words(size_t n) {
-  const size_t nb = (n + kSortTile - 1) / kSortTile;
-  const size_t hist = 256 * nb;
-  return hist + scan_scratch_words(hist) + 64;
+  const size_t tile = (size_t)kSortThreads * sort_items(n);
+  const size_t nb = (n + tile - 1) / tile;
+  return 256 * nb + 256 + 64;
 }
 
-void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
-                      uint32_t* scratch, hipStream_t s) {
-  if (n == 0) return;
-  if (bits < 0) bits = 0;
-  if (bits > 32) bits = 32;
-  const uint32_t nb = (uint32_t)((n + kSortTile - 1) / kSortTile);
+template <int ITEMS>
+static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
+                         uint32_t* scratch, hipStream_t s) {
+  constexpr size_t kTile = (size_t)kSortThreads * ITEMS;
+  const uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
   const int passes = bits <= 16 ? 2 : 4;
   int widths[4];
   int left = bits;
@@ -235,17 +302,16 @@ void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, si
     left -= w;
   }
   uint32_t* hist = scratch;
-  uint32_t* scan_scratch = scratch + 256 * (size_t)nb;
+  uint32_t* totals = scratch + 256 * (size_t)nb;
   uint32_t *ka = k0, *va = v0, *kb = k1, *vb = v1;
   int shift = 0;
   for (int p = 0; p < passes; ++p) {
     const int w = widths[p];
-    const size_t hn = (size_t)(1u << w) * nb;
-    hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka, n, shift, w,
-                       hist, nb);
-    scan_impl(hist, hist, hn, scan_scratch, true, s);
-    hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka,
-                       (const uint32_t*)va, kb, vb, n, shift, w, (const uint32_t*)hist, nb);
+    hipLaunchKernelGGL(radix_hist_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka, n, shift,
+                       w, hist, nb);
+    hipLaunchKernelGGL(radix_rowscan_kernel, dim3(1u << w), dim3(kScanThreads), 0, s, hist, nb, totals);
+    hipLaunchKernelGGL(radix_scatter_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka,
+                       (const uint32_t*)va, kb, vb, n, shift, w, (const uint32_t*)hist, (const uint32_t*)totals, nb);
     shift += w;
     uint32_t* t = ka;
     ka = kb;
@@ -254,6 +320,17 @@ void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, si
     va = vb;
     vb = t;
   }
+}
+
+void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
+                      uint32_t* scratch, hipStream_t s) {
+  if (n == 0) return;
+  if (bits < 0) bits = 0;
+  if (bits > 32) bits = 32;
+  if (sort_items(n) == 16)
+    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s);
+  else
+    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -270,34 +347,94 @@ void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, 
   hipLaunchKernelGGL(gather_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, s, src, idx, dst, n);
 }
 
+// Load-balanced emission.  The K output slots are cut into chunks of
+// kDupChunk; a first pass records, for every chunk, the depth-ordered
+// Gaussian whose output range contains the chunk's first slot.  Each
+// workgroup then fills exactly one chunk: it stages windows of up to 256
+// consecutive Gaussians (end offset, rectangle) in LDS and maps each of its
+// slots back to its Gaussian by binary search, so stores are coalesced and a
+// few huge splats cannot serialise a workgroup.
+constexpr uint32_t kDupChunk = 4096;
+
+__global__ __launch_bounds__(256) void chunk_first_kernel(int P, const uint32_t* __restrict__ offsets, uint32_t K,
+                                                          uint32_t* __restrict__ chunk_first) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P) return;
+  const uint32_t beg = (s == 0) ? 0u : min(offsets[s - 1], K);
+  const uint32_t end = min(offsets[s], K);
+  for (uint32_t c = (beg + kDupChunk - 1) / kDupChunk; c * kDupChunk < end; ++c) chunk_first[c] = (uint32_t)s;
+}
+
 __global__ __launch_bounds__(256) void duplicate_kernel(int P, const uint32_t* __restrict__ order,
                                                         const uint32_t* __restrict__ offsets,
                                                         const float4* __restrict__ splats, int gx, int gy,
                                                         uint32_t* __restrict__ tile_keys,
-                                                        uint32_t* __restrict__ values, uint32_t K) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= P) return;
-  const uint32_t g = order[s];
-  uint32_t off = (s == 0) ? 0u : offsets[s - 1];
-  const uint32_t end = min(offsets[s], K);
-  if (end <= off || g >= (uint32_t)P) return;  // culled: no tiles
-  const float4 s0 = splats[(size_t)DMGS_SPLAT_F4 * g];
-  const int r = __float_as_int(splats[(size_t)DMGS_SPLAT_F4 * g + 2].z);
-  int2 rmin, rmax;
-  tile_rect(make_float2(s0.x, s0.y), r, gx, gy, rmin, rmax);
-  for (int y = rmin.y; y < rmax.y; ++y)
-    for (int x = rmin.x; x < rmax.x && off < end; ++x) {
-      tile_keys[off] = (uint32_t)(y * gx + x);
-      values[off] = g;
-      ++off;
+                                                        uint32_t* __restrict__ values, uint32_t K,
+                                                        const uint32_t* __restrict__ chunk_first) {
+  __shared__ uint32_t s_end[256], s_g[256];
+  __shared__ int s_x0[256], s_y0[256], s_w[256];
+  const uint32_t lo_slot = blockIdx.x * kDupChunk;
+  const uint32_t hi_slot = min(K, lo_slot + kDupChunk);
+  uint32_t first = chunk_first[blockIdx.x];
+  while (true) {
+    // stage Gaussians first .. first+255 (those past P get an empty range)
+    const uint32_t s = first + threadIdx.x;
+    uint32_t end = 0xFFFFFFFFu, g = 0;
+    int x0 = 0, y0 = 0, w = 1;
+    if (s < (uint32_t)P) {
+      g = order[s];
+      const uint32_t beg = (s == 0) ? 0u : min(offsets[s - 1], K);
+      end = min(offsets[s], K);
+      if (end > beg && g < (uint32_t)P) {
+        const float4 s0 = splats[(size_t)DMGS_SPLAT_F4 * g];
+        const int r = __float_as_int(splats[(size_t)DMGS_SPLAT_F4 * g + 2].z);
+        int2 rmin, rmax;
+        tile_rect(make_float2(s0.x, s0.y), r, gx, gy, rmin, rmax);
+        x0 = rmin.x;
+        y0 = rmin.y;
+        w = max(1, rmax.x - rmin.x);
+      }
     }
+    s_end[threadIdx.x] = end;
+    s_g[threadIdx.x] = g;
+    s_x0[threadIdx.x] = x0;
+    s_y0[threadIdx.x] = y0;
+    s_w[threadIdx.x] = w;
+    __syncthreads();
+    const uint32_t win_begin = (first == 0) ? 0u : min(offsets[first - 1], K);
+    const uint32_t win_end = min(s_end[255], hi_slot);
+    const uint32_t o_begin = max(lo_slot, win_begin);
+    for (uint32_t o = o_begin + threadIdx.x; o < win_end; o += 256) {
+      int lo = 0, hi = 255;  // smallest j with s_end[j] > o
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_end[mid] > o) hi = mid;
+        else lo = mid + 1;
+      }
+      const uint32_t beg = (lo == 0) ? win_begin : s_end[lo - 1];
+      const uint32_t local = o - beg;
+      const uint32_t wj = (uint32_t)s_w[lo];
+      const uint32_t row = local / wj;
+      tile_keys[o] = ((uint32_t)s_y0[lo] + row) * (uint32_t)gx + (uint32_t)s_x0[lo] + (local - row * wj);
+      values[o] = s_g[lo];
+    }
+    const bool more = win_end < hi_slot && first + 256 < (uint32_t)P;
+    __syncthreads();
+    if (!more) break;
+    first += 256;
+  }
 }
 
+size_t duplicate_scratch_words(size_t K) { return (K + kDupChunk - 1) / kDupChunk + 64; }
+
 void launch_duplicate(int P, const uint32_t* order, const uint32_t* offsets, const float4* splats, int grid_x,
-                      int grid_y, uint32_t* tile_keys, uint32_t* values, uint32_t K, hipStream_t s) {
-  if (P <= 0) return;
-  hipLaunchKernelGGL(duplicate_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, order, offsets, splats, grid_x,
-                     grid_y, tile_keys, values, K);
+                      int grid_y, uint32_t* tile_keys, uint32_t* values, uint32_t K, uint32_t* scratch,
+                      hipStream_t s) {
+  if (P <= 0 || K == 0) return;
+  hipLaunchKernelGGL(chunk_first_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, offsets, K, scratch);
+  const uint32_t chunks = (K + kDupChunk - 1) / kDupChunk;
+  hipLaunchKernelGGL(duplicate_kernel, dim3(chunks), dim3(256), 0, s, P, order, offsets, splats, grid_x, grid_y,
+                     tile_keys, values, K, (const uint32_t*)scratch);
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, size_t n, uint32_t v) {

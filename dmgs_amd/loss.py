@@ -51,4 +51,45 @@ def ssim(img1, img2, window_size=11, size_average=True):
 
 
 def dssim_l1(image, gt, lambda_dssim=0.2):
+    """Reference formulation (torch ops, depthwise conv2d)."""
     return (1.0 - lambda_dssim) * l1_loss(image, gt) + lambda_dssim * (1.0 - ssim(image, gt))
+
+
+class _FusedDssimL1(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, image, gt, lambda_dssim):
+        from . import _native as N
+        img = image.detach().to(torch.float32).contiguous()
+        ref = gt.detach().to(device=img.device, dtype=torch.float32).contiguous()
+        if img.dim() == 4 and img.shape[0] == 1:
+            img, ref = img[0], ref.reshape(img.shape[1:])
+        if img.dim() != 3 or img.shape != ref.shape:
+            raise ValueError("fused_dssim_l1 expects two (C, H, W) images of the same shape")
+        C, H, W = img.shape
+        lib = N.load()
+        ws = torch.empty(lib.dmgs_dssim_l1_workspace_size(C, H, W), dtype=torch.uint8, device=img.device)
+        out = torch.empty(3, dtype=torch.float32, device=img.device)
+        N.check(lib.dmgs_dssim_l1_forward(N.ptr(img), N.ptr(ref), C, H, W, float(lambda_dssim), N.ptr(ws),
+                                          N.ptr(out), N.stream_handle(img.device)), "dmgs_dssim_l1_forward")
+        ctx.save_for_backward(img, ref, ws)
+        ctx.shape = image.shape
+        ctx.lam = float(lambda_dssim)
+        return out[0]
+
+    @staticmethod
+    def backward(ctx, grad_loss):
+        from . import _native as N
+        img, ref, ws = ctx.saved_tensors
+        C, H, W = img.shape
+        g = grad_loss.detach().to(torch.float32).reshape(1).contiguous()
+        dimg = torch.empty_like(img)
+        N.check(N.load().dmgs_dssim_l1_backward(N.ptr(img), N.ptr(ref), C, H, W, ctx.lam, N.ptr(ws), N.ptr(g),
+                                                N.ptr(dimg), N.stream_handle(img.device)),
+                "dmgs_dssim_l1_backward")
+        return dimg.reshape(ctx.shape), None, None
+
+
+def fused_dssim_l1(image, gt, lambda_dssim=0.2):
+    """(1 - lambda) L1 + lambda (1 - SSIM) as two gfx950 kernels (forward and
+    backward), same value and gradient as ``dssim_l1`` within FP32 rounding."""
+    return _FusedDssimL1.apply(image, gt, lambda_dssim)

@@ -149,6 +149,19 @@ int dmgs_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int6
 size_t dmgs_scan_scratch_size(int64_t n);
 int dmgs_inclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* scratch, dmgs_stream_t stream);
 
+/* ---- training objective ------------------------------------------------------ */
+/* Fused (1 - lambda) * L1 + lambda * (1 - SSIM) between img and gt, both
+ * (C, H, W) float32; SSIM as utils/loss_utils.py:17-62 (11x11 Gaussian window,
+ * sigma 1.5, zero padding, C1 = 1e-4, C2 = 9e-4, mean over C*H*W), objective as
+ * train_geo_stage3.py:122-124.  Forward writes (loss, l1, ssim) to loss_out
+ * (3 device floats) and keeps per-pixel coefficients in ws for the backward;
+ * backward writes dimg = dloss[0] * d(loss)/d(img). */
+size_t dmgs_dssim_l1_workspace_size(int32_t C, int32_t H, int32_t W);
+int dmgs_dssim_l1_forward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
+                          void* ws, float* loss_out, dmgs_stream_t stream);
+int dmgs_dssim_l1_backward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
+                           const void* ws, const float* dloss, float* dimg, dmgs_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
