@@ -44,6 +44,16 @@ _SIGNATURES = {
     "dmgs_mark_visible": (c_int32, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dmgs_sort_scratch_size": (c_size_t, [c_int64]),
     "dmgs_sort_pairs": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
+    "dmgs_bind_forward": (c_int32, [c_int64, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_int32, c_void_p,
+                                    c_void_p, c_float, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dmgs_bind_backward": (c_int32, [c_int64, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_int32, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
+    "dmgs_bind_cov_forward": (c_int32, [c_int64, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_int32, c_void_p,
+                                        c_float, c_float, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dmgs_bind_cov_backward": (c_int32, [c_int64, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_int32,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dmgs_faces_in_frustum": (c_int32, [c_int64, c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dmgs_dssim_l1_workspace_size": (c_size_t, [c_int32, c_int32, c_int32]),
     "dmgs_dssim_l1_forward": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_float, c_void_p, c_void_p,
                                         c_void_p]),

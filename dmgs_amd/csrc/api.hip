@@ -515,3 +515,134 @@ int dmgs_dssim_l1_backward(const float* img, const float* gt, int32_t C, int32_t
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// mesh binding
+// ---------------------------------------------------------------------------
+namespace {
+int check_mesh(int64_t V, const float* verts, int64_t F, const int32_t* faces, int32_t G, const float* bc) {
+  if (V <= 0 || V > (int64_t)0x7fffffff / 3) return fail(DMGS_EINVAL, "invalid vertex count %lld", (long long)V);
+  if (F < 0 || G <= 0 || F * (int64_t)G > (int64_t)0x7fffffff)
+    return fail(DMGS_EINVAL, "invalid face count %lld or Gaussians per face %d", (long long)F, G);
+  if (!verts || !faces || !bc) return fail(DMGS_EINVAL, "verts, faces and barycentrics are required");
+  return DMGS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int dmgs_bind_forward(int64_t V, const float* verts, int64_t F, const int32_t* faces, int32_t G, const float* bc,
+                      int32_t bc_per_gaussian, const float* rot2d, const float* scale2d, float thin_z, float* xyz,
+                      float* rot, float* scale, dmgs_stream_t stream) {
+  int rc = check_mesh(V, verts, F, faces, G, bc);
+  if (rc) return rc;
+  if (!xyz) return fail(DMGS_EINVAL, "xyz output is required");
+  if ((rot2d == nullptr) != (rot == nullptr) || (scale2d == nullptr) != (scale == nullptr))
+    return fail(DMGS_EINVAL, "rotation / scale inputs and outputs must be given together");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  dmgs::BindArgs a{};
+  a.F = (int)F;
+  a.G = G;
+  a.verts = verts;
+  a.faces = faces;
+  a.bc = bc;
+  a.bc_per_gaussian = bc_per_gaussian;
+  a.rot2d = rot2d;
+  a.scale2d = scale2d;
+  a.thin_z = thin_z;
+  a.xyz = xyz;
+  a.rot = rot;
+  a.scale = scale;
+  dmgs::launch_bind_fwd(a, st);
+  return check_launch("bind forward", st, false);
+}
+
+int dmgs_bind_backward(int64_t V, const float* verts, int64_t F, const int32_t* faces, int32_t G, const float* bc,
+                       int32_t bc_per_gaussian, const float* rot2d, const float* scale2d, const float* dxyz,
+                       const float* drot, const float* dscale, float* dverts, float* drot2d, float* dscale2d,
+                       dmgs_stream_t stream) {
+  int rc = check_mesh(V, verts, F, faces, G, bc);
+  if (rc) return rc;
+  if (!dxyz || !dverts) return fail(DMGS_EINVAL, "dxyz and dverts are required");
+  if (drot && (!rot2d || !drot2d)) return fail(DMGS_EINVAL, "drot needs rot2d and drot2d");
+  if (dscale && (!scale2d || !dscale2d)) return fail(DMGS_EINVAL, "dscale needs scale2d and dscale2d");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  dmgs::BindArgs a{};
+  a.F = (int)F;
+  a.G = G;
+  a.verts = verts;
+  a.faces = faces;
+  a.bc = bc;
+  a.bc_per_gaussian = bc_per_gaussian;
+  a.rot2d = rot2d;
+  a.scale2d = scale2d;
+  a.dxyz = dxyz;
+  a.drot = drot;
+  a.dscale = dscale;
+  a.dverts = dverts;
+  a.drot2d = drot2d;
+  a.dscale2d = dscale2d;
+  dmgs::launch_bind_bwd(a, st);
+  return check_launch("bind backward", st, false);
+}
+
+int dmgs_bind_cov_forward(int64_t V, const float* verts, int64_t F, const int32_t* faces, int32_t G, const float* bc,
+                          int32_t bc_per_gaussian, const float* k, float rad_base, float flat_scale, int32_t adaptive,
+                          float* xyz, float* cov, float* Lmat, dmgs_stream_t stream) {
+  int rc = check_mesh(V, verts, F, faces, G, bc);
+  if (rc) return rc;
+  if (!k || !xyz || !cov || !Lmat) return fail(DMGS_EINVAL, "k, xyz, cov and Lmat are required");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  dmgs::BindArgs a{};
+  a.F = (int)F;
+  a.G = G;
+  a.verts = verts;
+  a.faces = faces;
+  a.bc = bc;
+  a.bc_per_gaussian = bc_per_gaussian;
+  a.k = k;
+  a.rad_base = rad_base;
+  a.flat_scale = flat_scale;
+  a.adaptive = adaptive;
+  a.xyz = xyz;
+  a.cov = cov;
+  a.Lmat = Lmat;
+  dmgs::launch_bind_cov_fwd(a, st);
+  return check_launch("bind cov forward", st, false);
+}
+
+int dmgs_bind_cov_backward(int64_t V, const float* verts, int64_t F, const int32_t* faces, int32_t G,
+                           const float* bc, int32_t bc_per_gaussian, const float* k, const float* Lmat,
+                           const float* dxyz, const float* dcov, float* dverts, float* dk, dmgs_stream_t stream) {
+  int rc = check_mesh(V, verts, F, faces, G, bc);
+  if (rc) return rc;
+  if (!k || !Lmat || !dxyz || !dcov || !dverts) return fail(DMGS_EINVAL, "missing buffers");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  dmgs::BindArgs a{};
+  a.F = (int)F;
+  a.G = G;
+  a.verts = verts;
+  a.faces = faces;
+  a.bc = bc;
+  a.bc_per_gaussian = bc_per_gaussian;
+  a.k = k;
+  a.Lmat = const_cast<float*>(Lmat);
+  a.dxyz = dxyz;
+  a.dcov = dcov;
+  a.dverts = dverts;
+  a.dk = dk;
+  dmgs::launch_bind_bwd(a, st);
+  return check_launch("bind cov backward", st, false);
+}
+
+int dmgs_faces_in_frustum(int64_t V, const float* verts, int64_t F, const int32_t* faces, const float* projmatrix,
+                          uint8_t* mask, dmgs_stream_t stream) {
+  if (V <= 0 || F < 0 || F > (int64_t)0x7fffffff) return fail(DMGS_EINVAL, "invalid sizes");
+  if (F == 0) return DMGS_OK;
+  if (!verts || !faces || !projmatrix || !mask) return fail(DMGS_EINVAL, "null pointer");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  dmgs::launch_faces_in_frustum((int)F, verts, faces, projmatrix, mask, st);
+  return check_launch("faces in frustum", st, false);
+}
+
+}  // extern "C"

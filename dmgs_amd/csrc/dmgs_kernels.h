@@ -104,6 +104,43 @@ struct GeometryBwdArgs {
 };
 void launch_geometry_bwd(const GeometryBwdArgs& a, hipStream_t s);
 
+// ---- mesh binding ----
+struct BindArgs {
+  int F, G;                 // faces, Gaussians per face
+  const float* verts;       // (V, 3)
+  const int32_t* faces;     // (F, 3)
+  const float* bc;          // (G, 3) shared, or (F*G, 3) per Gaussian
+  int bc_per_gaussian;
+  // stage 3 parameters (may be null)
+  const float* rot2d;       // (F*G, 2)
+  const float* scale2d;     // (F*G, 2) log-scales
+  float thin_z;
+  // stage 2 parameters
+  const float* k;           // device scalar tanh(scale_factor) * max_scale
+  float rad_base, flat_scale;
+  int adaptive;
+  // forward outputs
+  float* xyz;               // (F*G, 3)
+  float* rot;               // (F*G, 4)
+  float* scale;             // (F*G, 3)
+  float* cov;               // (F*G, 6)
+  float* Lmat;              // (F, 9) stage-2 triangle matrices (written by the forward)
+  // backward inputs / outputs
+  const float* dxyz;
+  const float* drot;
+  const float* dscale;
+  const float* dcov;
+  float* dverts;            // (V, 3), accumulated
+  float* drot2d;
+  float* dscale2d;
+  float* dk;                // device scalar, accumulated
+};
+void launch_bind_fwd(const BindArgs& a, hipStream_t s);
+void launch_bind_cov_fwd(const BindArgs& a, hipStream_t s);
+void launch_bind_bwd(const BindArgs& a, hipStream_t s);
+void launch_faces_in_frustum(int F, const float* verts, const int32_t* faces, const float* proj, uint8_t* mask,
+                             hipStream_t s);
+
 // ---- fused L1 + SSIM loss ----
 size_t dssim_partials(int C, int H, int W);
 void launch_dssim_forward(const float* x, const float* y, int C, int H, int W, const float* w11, float lam,

@@ -149,6 +149,46 @@ int dmgs_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int6
 size_t dmgs_scan_scratch_size(int64_t n);
 int dmgs_inclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* scratch, dmgs_stream_t stream);
 
+/* ---- mesh -> Gaussian binding ------------------------------------------------ */
+/* Shared arguments: verts (V,3) float32, faces (F,3) int32, G Gaussians per
+ * face in face-major order (Gaussian f*G + j), barycentrics bc either (G,3)
+ * shared by every face (bc_per_gaussian = 0, as generate_barycentric_v2 gives,
+ * geo/mesh_utils.py:15-40) or (F*G,3) (bc_per_gaussian = 1).
+ *
+ * Stage 3, replaces GaussianGeoModel.renew_gaussian + get_rotation /
+ * get_scaling (scene/gaussian_geo_model_finetune.py:396-421, 445-482):
+ * xyz (F*G,3); rot (F*G,4) = normalize(matrix_to_quaternion(R_t2w R_t(rot2d)));
+ * scale (F*G,3) = (exp(scale2d), thin_z).  rot2d/rot and scale2d/scale are
+ * optional in pairs. */
+int dmgs_bind_forward(int64_t V, const float* verts, int64_t F, const int32_t* faces, int32_t G, const float* bc,
+                      int32_t bc_per_gaussian, const float* rot2d, const float* scale2d, float thin_z, float* xyz,
+                      float* rot, float* scale, dmgs_stream_t stream);
+/* Gradients of the stage-3 binding: dverts (V,3) is ADDED to (clear it first);
+ * drot2d, dscale2d (F*G,2) are written. */
+int dmgs_bind_backward(int64_t V, const float* verts, int64_t F, const int32_t* faces, int32_t G, const float* bc,
+                       int32_t bc_per_gaussian, const float* rot2d, const float* scale2d, const float* dxyz,
+                       const float* drot, const float* dscale, float* dverts, float* drot2d, float* dscale2d,
+                       dmgs_stream_t stream);
+/* Stage 2, replaces GaussianGeoModel.renew_gaussian's means and
+ * get_covariance_dyn (scene/gaussian_geo_model_mlp_flex.py:258-310, 370-385):
+ * xyz (F*G,3); cov (F*G,6) = (R_t2w L k)(R_t2w L k)^T per face with
+ * k = *k (device scalar, tanh(scale_factor) * max_scale) and L the affine
+ * triangle matrix (no gradient), written to Lmat (F,9) for the backward;
+ * rad_base scales |v1 - v0|, flat_scale is L[2][2] (spatial_lr_scale * 1e-6),
+ * adaptive selects the affine M (else identity). */
+int dmgs_bind_cov_forward(int64_t V, const float* verts, int64_t F, const int32_t* faces, int32_t G, const float* bc,
+                          int32_t bc_per_gaussian, const float* k, float rad_base, float flat_scale, int32_t adaptive,
+                          float* xyz, float* cov, float* Lmat, dmgs_stream_t stream);
+/* Gradients of the stage-2 binding: dverts (V,3) and dk (device scalar,
+ * dL/dk) are ADDED to. */
+int dmgs_bind_cov_backward(int64_t V, const float* verts, int64_t F, const int32_t* faces, int32_t G,
+                           const float* bc, int32_t bc_per_gaussian, const float* k, const float* Lmat,
+                           const float* dxyz, const float* dcov, float* dverts, float* dk, dmgs_stream_t stream);
+/* mask[f] = 1 if face f's centroid projects inside |ndc| < 1.05 with w > 0
+ * (in_frustum, scene/gaussian_geo_model_finetune.py:33-48). */
+int dmgs_faces_in_frustum(int64_t V, const float* verts, int64_t F, const int32_t* faces, const float* projmatrix,
+                          uint8_t* mask, dmgs_stream_t stream);
+
 /* ---- training objective ------------------------------------------------------ */
 /* Fused (1 - lambda) * L1 + lambda * (1 - SSIM) between img and gt, both
  * (C, H, W) float32; SSIM as utils/loss_utils.py:17-62 (11x11 Gaussian window,

@@ -1,0 +1,129 @@
+"""GPU parity of the mesh binding kernels against the FP64 oracle, alone and
+chained through the rasterizer into the mesh vertices."""
+import pytest
+import torch
+
+from dmgs_amd import binding as PB
+from dmgs_amd import cameras as C
+from dmgs_amd import scenes as S
+from dmgs_amd.gaussian_renderer import PipelineParams, render
+from oracle import binding as OB
+from oracle import raster as OR
+from tests.helpers import grad_rel_err, oracle_settings, settings_from_cam
+
+pytestmark = pytest.mark.gpu
+F64 = torch.float64
+
+
+def _mesh(sub, seed, G=6, per_gaussian=True):
+    g = torch.Generator().manual_seed(seed)
+    v, f = S.icosphere(sub)
+    v = v + torch.randn(v.shape, generator=g) * 0.02
+    n = f.shape[0] * G
+    if per_gaussian:
+        e = -torch.log(torch.rand(n, 3, generator=g))
+        bc = e / e.sum(1, keepdim=True)
+    else:
+        bc = PB.barycentric_v2(G)[0]
+    rot2d = torch.randn(n, 2, generator=g)
+    scale2d = torch.randn(n, 2, generator=g) * 0.3 - 3
+    return v, f, bc, rot2d, scale2d
+
+
+@pytest.mark.parametrize("per_gaussian", [True, False])
+def test_stage3_forward_backward(per_gaussian):
+    print(f"case: bind stage3 per_gaussian={per_gaussian}", flush=True)
+    v, f, bc, rot2d, scale2d = _mesh(3, 1, per_gaussian=per_gaussian)
+    n = rot2d.shape[0]
+    ov, orot, osc = [t.double().requires_grad_(True) for t in (v, rot2d, scale2d)]
+    xyz_o, q_o, s_o, R_o = OB.stage3(ov, f, bc.double(), orot, osc, 1e-4)
+    gv, grot, gsc = [t.cuda().requires_grad_(True) for t in (v, rot2d, scale2d)]
+    xyz, q, s = PB.bind_stage3(gv, f.cuda(), bc.cuda(), grot, gsc, 1e-4)
+    assert (xyz.cpu().double() - xyz_o).abs().max() < 1e-5
+    assert (s.cpu().double() - s_o).abs().max() < 1e-6
+    # the sign of q is a convention; R(q) is what the rasterizer uses
+    qq = q.detach().cpu().double()
+    assert (OB.quat_to_rot(qq) - R_o).abs().max() < 1e-5
+    sign = torch.sign((qq * q_o).sum(1, keepdim=True))
+    assert (qq - sign * q_o.detach()).abs().max() < 1e-5
+    g = torch.Generator().manual_seed(2)
+    wx, wq, ws = torch.randn(n, 3, generator=g), torch.randn(n, 4, generator=g), torch.randn(n, 3, generator=g)
+    ref = torch.autograd.grad((xyz_o * wx).sum() + (q_o * (sign * wq)).sum() + (s_o * ws).sum(), [ov, orot, osc])
+    got = torch.autograd.grad((xyz * wx.cuda()).sum() + (q * wq.cuda()).sum() + (s * ws.cuda()).sum(),
+                              [gv, grot, gsc])
+    for name, a, b in zip(("verts", "rotation", "scaling"), got, ref):
+        e = grad_rel_err(a, b)
+        print(f"  {name}: rel err {e:.2e}")
+        assert e < 1e-3
+
+
+def test_stage2_forward_backward():
+    print("case: bind stage2", flush=True)
+    v, f, _, _, _ = _mesh(3, 5)
+    bc, rad = PB.barycentric_v2(6)
+    sf = torch.tensor([0.4])
+    ov, osf = v.double().requires_grad_(True), sf.double().requires_grad_(True)
+    k_o = torch.tanh(osf.reshape(())) * 2.0
+    xyz_o, cov_o, L_o = OB.stage2(ov, f, bc.double(), 6, k_o, rad, 2.5e-6)
+    gv, gsf = v.cuda().requires_grad_(True), sf.cuda().requires_grad_(True)
+    xyz, cov, L = PB.bind_stage2(gv, f.cuda(), bc.cuda(), gsf, rad, max_scale=2.0, spatial_lr_scale=2.5)
+    assert (xyz.cpu().double() - xyz_o).abs().max() < 1e-5
+    assert grad_rel_err(cov.detach(), cov_o.detach()) < 1e-5
+    assert grad_rel_err(L, L_o) < 1e-5
+    g = torch.Generator().manual_seed(3)
+    wx, wc = torch.randn(xyz.shape, generator=g), torch.randn(cov.shape, generator=g) * 1e4
+    ref = torch.autograd.grad((xyz_o * wx).sum() + (cov_o * wc).sum(), [ov, osf])
+    got = torch.autograd.grad((xyz * wx.cuda()).sum() + (cov * wc.cuda()).sum(), [gv, gsf])
+    for name, a, b in zip(("verts", "scale_factor"), got, ref):
+        e = grad_rel_err(a, b)
+        print(f"  {name}: rel err {e:.2e}")
+        assert e < 1e-3
+
+
+def test_faces_in_frustum():
+    print("case: faces in frustum", flush=True)
+    cam = C.front_camera(64, 64, device="cpu")
+    v, f = S.icosphere(3)
+    v = v * 3.0
+    ref = OB.in_frustum(cam.full_proj_transform.double(), v.double(), f)
+    got = PB.faces_in_frustum(v.cuda(), f.cuda(), cam.full_proj_transform.cuda()).cpu()
+    assert (got == ref).float().mean() > 0.999
+
+
+def test_mesh_bound_render_into_vertices():
+    """Binding -> drop-in render -> backward into the vertices and SH,
+    against the oracle chain (oracle rasterizer gradients pushed through the
+    FP64 binding by autograd)."""
+    print("case: mesh-bound render", flush=True)
+    W, H = 120, 96
+    model = PB.mesh_scene(subdivisions=3, seed=0)
+    cam = C.front_camera(W, H, device="cpu")
+    bg = torch.ones(3)
+    g = torch.Generator().manual_seed(5)
+    dL = torch.randn(3, H, W, generator=g)
+    # oracle chain
+    v = model.verts.double().requires_grad_(True)
+    xyz_o, q_o, s_o, _ = OB.stage3(v, model.faces, model.bc_coords.double(), model._rotation.double(),
+                                   model._scaling.double(), model.thin_z_scale)
+    feats = torch.cat([model._features_dc, model._features_rest], 1)
+    opac = torch.sigmoid(model._opacity)
+    rs = settings_from_cam(cam, bg, 3, device="cpu")
+    ors = oracle_settings(rs)
+    ocol, _, ost = OR.rasterize(xyz_o.detach(), opac, ors, shs=feats, scales=s_o.detach(), rotations=q_o.detach())
+    dL[:, ost["ambiguous"]] = 0
+    og = OR.rasterize_backward(ost, ors, xyz_o.detach(), dL.double(), shs=feats, scales=s_o.detach(),
+                               rotations=q_o.detach())
+    (dv_ref,) = torch.autograd.grad([xyz_o, q_o], [v], [og[0], og[6]])   # scales do not depend on verts
+    # device chain
+    gm = PB.mesh_scene(subdivisions=3, seed=0, device="cuda").requires_grad_(True)
+    gm.renew_gaussian()
+    out = render(C.front_camera(W, H, device="cuda"), gm, PipelineParams(), bg.cuda())
+    ok = ~ost["ambiguous"]
+    err = (out["render"].detach().cpu().double() - ocol).abs().amax(0)[ok].max().item()
+    print(f"  image max err {err:.2e}")
+    assert err < 1e-4
+    out["render"].backward(dL.cuda())
+    e_v = grad_rel_err(gm.verts.grad, dv_ref)
+    e_sh = grad_rel_err(gm._features_rest.grad, og[2][:, 1:, :])
+    print(f"  verts grad rel err {e_v:.2e}, SH grad rel err {e_sh:.2e}")
+    assert e_v < 1e-3 and e_sh < 1e-3
