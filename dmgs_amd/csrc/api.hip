@@ -275,6 +275,7 @@ int dmgs_render_forward(const dmgs_settings* s, int64_t P, int64_t K, const void
   r.H = s->image_height;
   r.grid_x = gx;
   r.grid_y = gy;
+  r.num_tiles = gx * gy;
   r.P = (uint32_t)P;
   r.K = (uint32_t)K;
   r.ranges = at<uint2>(image_ws, IL.ranges);
@@ -334,6 +335,7 @@ int dmgs_render_backward(const dmgs_settings* s, int64_t P, int64_t K, const voi
   r.H = s->image_height;
   r.grid_x = gx;
   r.grid_y = gy;
+  r.num_tiles = gx * gy;
   r.P = (uint32_t)P;
   r.K = (uint32_t)K;
   r.ranges = at<uint2>(image_ws, IL.ranges);

@@ -33,6 +33,19 @@
 
 namespace dmgs {
 
+// One camera of a multi-view launch (host-filled, copied to the device).
+struct ViewDesc {
+  float view[16];       // world_view_transform, reference storage order
+  float proj[16];       // full_proj_transform
+  float campos[4];      // xyz + pad
+  float bg[4];          // rgb + pad
+  float tanfovx, tanfovy, focal_x, focal_y;
+  int32_t W, H, grid_x, grid_y;
+  int32_t tile_base;    // first global tile id of the view
+  int32_t pad;
+  int64_t pixel_base;   // first pixel of the view in the packed per-pixel buffers
+};
+
 __device__ __forceinline__ float3 xform_point_4x3(const float3 p, const float* m) {
   return make_float3(m[0] * p.x + m[4] * p.y + m[8] * p.z + m[12],
                      m[1] * p.x + m[5] * p.y + m[9] * p.z + m[13],

@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "dmgs_common.h"
+
 namespace dmgs {
 
 struct PreprocessArgs {
@@ -58,9 +60,16 @@ void launch_duplicate(int P, const uint32_t* order, const uint32_t* offsets, con
 void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s);
 
 // ---- blend ----
+// Single view: W, H, grid_x, grid_y, bg and the image pointers describe the
+// view.  Several views: `views` / `tile_view` map each global tile to its
+// view; images, final_T and n_contrib are packed at each view's pixel_base.
 struct RenderArgs {
   int W, H, grid_x, grid_y;
-  uint32_t P, K;  // Gaussian count and list length (guards for the gathers)
+  int num_tiles;          // tiles in the launch (all views)
+  uint32_t P, K;          // splat records and list capacity (guards for the gathers)
+  const uint32_t* K_dev;  // optional device list length (clamped to K)
+  const ViewDesc* views;  // optional view table (multi-view)
+  const int32_t* tile_view;
   const uint2* ranges;
   const uint32_t* point_list;
   const float4* splats;
@@ -72,8 +81,11 @@ struct RenderArgs {
   const float* dL_dpix;  // (3, H, W)
   float* gacc;           // (P, DMGS_GACC_STRIDE)
 };
-void launch_render_fwd(const RenderArgs& a, hipStream_t s);
-void launch_render_bwd(const RenderArgs& a, hipStream_t s);
+// Work-group shape / LDS batch depth of the blend kernels (results are the
+// same for every variant; the autotuner picks the fastest).
+enum RenderVariant { kRender16x256 = 0, kRender16x128 = 1, kRender8x64 = 2, kRender8x128 = 3, kRenderVariants = 4 };
+void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
+void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
 
 // ---- geometry backward ----
 struct GeometryBwdArgs {

@@ -7,29 +7,36 @@
 // alpha = min(0.99, o exp(power)), skip if alpha < 1/255, stop (without
 // adding) once T (1 - alpha) < 1e-4, C += c alpha T; output C + T bg.
 //
-// Layout: one workgroup of 256 lanes (4 wave64) per 16x16 tile; Gaussian
-// batches of 256 splat records are staged in LDS with one coalesced 48-byte
-// gather per lane.
+// Layout: a workgroup covers SUB x SUB pixels of a 16x16 binning tile (SUB =
+// 16: the whole tile with 4 wave64; SUB = 8: one wave64 per quarter tile, four
+// workgroups reading the same list).  Splat records are staged in LDS in
+// batches of BATCH with one 48-byte gather per record.  Every variant
+// evaluates each pixel in the same order, so they give identical images.
 //
 // Backward: the list is walked from the back, starting at the largest
-// n_contrib of the tile (entries behind every pixel's last contributor
+// n_contrib of the workgroup (entries behind every pixel's last contributor
 // cannot contribute), transmittance is recomputed by division instead of
-// being stored, and each wave reduces its 9 per-Gaussian partial gradients
-// with DPP row sums into a per-batch LDS accumulator; the workgroup flushes
-// that accumulator with one global atomic per (Gaussian, value) per batch.
+// being stored, and each wave reduces its 9 per-splat partial gradients with
+// DPP row sums into a per-batch LDS accumulator; the workgroup flushes that
+// accumulator with one global atomic per (splat, value) per batch.
+//
+// Several views in one launch: a global tile id indexes the ranges; the view
+// table gives each tile's image size, background and packed output offset.
 #include "dmgs_common.h"
 #include "dmgs_kernels.h"
 
 namespace dmgs {
 
+template <int BATCH>
 struct SplatLDS {
-  float2 xy[DMGS_BLOCK_SIZE];
-  float4 con_o[DMGS_BLOCK_SIZE];  // conic a, b, c, opacity
-  float4 rgb_id[DMGS_BLOCK_SIZE]; // r, g, b, id (bits)
+  float2 xy[BATCH];
+  float4 con_o[BATCH];   // conic a, b, c, opacity
+  float4 rgb_id[BATCH];  // r, g, b, id (bits)
 };
 
-__device__ __forceinline__ void stage_splat(SplatLDS& L, int slot, const float4* __restrict__ splats, uint32_t id,
-                                            uint32_t P) {
+template <int BATCH>
+__device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const float4* __restrict__ splats,
+                                            uint32_t id, uint32_t P) {
   if (id >= P) {  // cannot happen for lists built by the binning step; a zero-opacity entry never contributes
     L.xy[slot] = make_float2(0.f, 0.f);
     L.con_o[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -45,31 +52,83 @@ __device__ __forceinline__ void stage_splat(SplatLDS& L, int slot, const float4*
   L.rgb_id[slot] = make_float4(s1.z, s1.w, s2.x, __uint_as_float(id));
 }
 
-__global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_fwd_kernel(RenderArgs a) {
-  __shared__ SplatLDS L;
-  const int tile = blockIdx.x;
-  const int tx = tile % a.grid_x, ty = tile / a.grid_x;
-  const int lx = threadIdx.x & (DMGS_BLOCK_X - 1), ly = threadIdx.x / DMGS_BLOCK_X;
-  const int px = tx * DMGS_BLOCK_X + lx, py = ty * DMGS_BLOCK_Y + ly;
-  const bool inside = px < a.W && py < a.H;
-  const float pxf = (float)px, pyf = (float)py;
-  uint2 range = a.ranges[tile];
-  range.y = min(range.y, a.K);
-  range.x = min(range.x, range.y);
-  const int todo_total = (int)(range.y - range.x);
-  const int rounds = (todo_total + DMGS_BLOCK_SIZE - 1) / DMGS_BLOCK_SIZE;
+// Everything a workgroup needs to know about its pixels.
+struct TileCtx {
+  int tile;            // global tile id (indexes ranges)
+  int px, py;          // this lane's pixel
+  int W, H;
+  bool inside;
+  size_t pix;          // index into the per-pixel buffers
+  size_t plane;        // W * H of the view
+  size_t img;          // offset of the view's colour planes
+  float bg0, bg1, bg2;
+  uint2 range;
+};
 
-  bool done = !inside;
+template <int SUB>
+__device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
+  constexpr int Q = DMGS_BLOCK_X / SUB;  // workgroups per tile edge
+  TileCtx c;
+  c.tile = blockIdx.x / (Q * Q);
+  const int q = blockIdx.x - c.tile * Q * Q;
+  int lt = c.tile, gx = a.grid_x;
+  int64_t pbase = 0;
+  if (a.views) {
+    const ViewDesc& v = a.views[a.tile_view[c.tile]];
+    lt = c.tile - v.tile_base;
+    gx = v.grid_x;
+    c.W = v.W;
+    c.H = v.H;
+    c.bg0 = v.bg[0];
+    c.bg1 = v.bg[1];
+    c.bg2 = v.bg[2];
+    pbase = v.pixel_base;
+  } else {
+    c.W = a.W;
+    c.H = a.H;
+    c.bg0 = a.bg[0];
+    c.bg1 = a.bg[1];
+    c.bg2 = a.bg[2];
+  }
+  const int tx = lt % gx, ty = lt / gx;
+  const int lx = threadIdx.x % SUB, ly = threadIdx.x / SUB;
+  c.px = tx * DMGS_BLOCK_X + (q % Q) * SUB + lx;
+  c.py = ty * DMGS_BLOCK_Y + (q / Q) * SUB + ly;
+  c.inside = c.px < c.W && c.py < c.H;
+  c.plane = (size_t)c.W * c.H;
+  c.pix = (size_t)pbase + (size_t)c.py * c.W + c.px;
+  c.img = 3 * (size_t)pbase;
+  uint32_t K = a.K;
+  if (a.K_dev) K = min(K, *a.K_dev);
+  uint2 r = a.ranges[c.tile];
+  r.y = min(r.y, K);
+  r.x = min(r.x, r.y);
+  c.range = r;
+  return c;
+}
+
+template <int SUB, int BATCH>
+__global__ __launch_bounds__(SUB* SUB) void render_fwd_kernel(RenderArgs a) {
+  constexpr int NT = SUB * SUB;
+  __shared__ SplatLDS<BATCH> L;
+  const TileCtx c = tile_ctx<SUB>(a);
+  const float pxf = (float)c.px, pyf = (float)c.py;
+  const int todo_total = (int)(c.range.y - c.range.x);
+  const int rounds = (todo_total + BATCH - 1) / BATCH;
+
+  bool done = !c.inside;
   float T = 1.0f;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f;
   uint32_t contributor = 0, last_contributor = 0;
 
   for (int r = 0; r < rounds; ++r) {
-    if (__syncthreads_count(done) == DMGS_BLOCK_SIZE) break;
-    const int pos = r * DMGS_BLOCK_SIZE + threadIdx.x;
-    if (pos < todo_total) stage_splat(L, threadIdx.x, a.splats, a.point_list[range.x + pos], a.P);
+    if (__syncthreads_count(done) == NT) break;
+    for (int s = threadIdx.x; s < BATCH; s += NT) {
+      const int pos = r * BATCH + s;
+      if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P);
+    }
     __syncthreads();
-    const int n = min(DMGS_BLOCK_SIZE, todo_total - r * DMGS_BLOCK_SIZE);
+    const int n = min(BATCH, todo_total - r * BATCH);
     for (int j = 0; !done && j < n; ++j) {
       ++contributor;
       const float2 xy = L.xy[j];
@@ -84,63 +143,60 @@ __global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_fwd_kernel(RenderArgs 
         done = true;
         continue;
       }
-      const float4 c = L.rgb_id[j];
+      const float4 col = L.rgb_id[j];
       const float w = alpha * T;
-      C0 += c.x * w;
-      C1 += c.y * w;
-      C2 += c.z * w;
+      C0 += col.x * w;
+      C1 += col.y * w;
+      C2 += col.z * w;
       T = test_T;
       last_contributor = contributor;
     }
   }
-  if (inside) {
-    const int pid = py * a.W + px;
-    const int HW = a.W * a.H;
-    a.final_T[pid] = T;
-    a.n_contrib[pid] = last_contributor;
-    a.out_color[pid] = C0 + T * a.bg[0];
-    a.out_color[HW + pid] = C1 + T * a.bg[1];
-    a.out_color[2 * HW + pid] = C2 + T * a.bg[2];
+  if (c.inside) {
+    const size_t pid = c.pix - c.img / 3;  // pixel index within the view
+    a.final_T[c.pix] = T;
+    a.n_contrib[c.pix] = last_contributor;
+    float* out = a.out_color + c.img;
+    out[pid] = C0 + T * c.bg0;
+    out[c.plane + pid] = C1 + T * c.bg1;
+    out[2 * c.plane + pid] = C2 + T * c.bg2;
   }
 }
 
-__global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_bwd_kernel(RenderArgs a) {
-  __shared__ SplatLDS L;
-  __shared__ uint32_t s_max[DMGS_BLOCK_SIZE / DMGS_WAVE];
-  __shared__ float s_acc[DMGS_BLOCK_SIZE][DMGS_GACC_USED];
-  const int tile = blockIdx.x;
-  const int tx = tile % a.grid_x, ty = tile / a.grid_x;
-  const int lx = threadIdx.x & (DMGS_BLOCK_X - 1), ly = threadIdx.x / DMGS_BLOCK_X;
-  const int px = tx * DMGS_BLOCK_X + lx, py = ty * DMGS_BLOCK_Y + ly;
-  const bool inside = px < a.W && py < a.H;
-  const float pxf = (float)px, pyf = (float)py;
-  uint2 range = a.ranges[tile];
-  range.y = min(range.y, a.K);
-  range.x = min(range.x, range.y);
-  const int HW = a.W * a.H;
-  const int pid = py * a.W + px;
+template <int SUB, int BATCH>
+__global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
+  constexpr int NT = SUB * SUB;
+  constexpr int NW = NT / DMGS_WAVE;
+  __shared__ SplatLDS<BATCH> L;
+  __shared__ uint32_t s_max[NW];
+  __shared__ float s_acc[BATCH][DMGS_GACC_USED];
+  const TileCtx c = tile_ctx<SUB>(a);
+  const float pxf = (float)c.px, pyf = (float)c.py;
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-  const float T_final = inside ? a.final_T[pid] : 0.0f;
-  const uint32_t last_contributor = inside ? a.n_contrib[pid] : 0u;
-  float g0 = 0.f, g1 = 0.f, g2 = 0.f;
-  if (inside) {
-    g0 = a.dL_dpix[pid];
-    g1 = a.dL_dpix[HW + pid];
-    g2 = a.dL_dpix[2 * HW + pid];
+  float T_final = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f;
+  uint32_t last_contributor = 0;
+  if (c.inside) {
+    const size_t pid = c.pix - c.img / 3;
+    const float* dp = a.dL_dpix + c.img;
+    T_final = a.final_T[c.pix];
+    last_contributor = a.n_contrib[c.pix];
+    g0 = dp[pid];
+    g1 = dp[c.plane + pid];
+    g2 = dp[2 * c.plane + pid];
   }
-  const float bg_dot = a.bg[0] * g0 + a.bg[1] * g1 + a.bg[2] * g2;
-  const float ddelx_dx = 0.5f * (float)a.W, ddely_dy = 0.5f * (float)a.H;
+  const float bg_dot = c.bg0 * g0 + c.bg1 * g1 + c.bg2 * g2;
+  const float ddelx_dx = 0.5f * (float)c.W, ddely_dy = 0.5f * (float)c.H;
 
-  // Entries at list positions >= max(n_contrib) of the tile contribute nothing.
+  // Entries at list positions >= max(n_contrib) of the workgroup contribute nothing.
   uint32_t m = last_contributor;
   for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
   if (lane == 0) s_max[wave] = m;
   __syncthreads();
   uint32_t tile_max = 0;
-  for (int w = 0; w < DMGS_BLOCK_SIZE / DMGS_WAVE; ++w) tile_max = max(tile_max, s_max[w]);
-  const int todo_total = min((int)(range.y - range.x), (int)tile_max);
-  const int rounds = (todo_total + DMGS_BLOCK_SIZE - 1) / DMGS_BLOCK_SIZE;
+  for (int w = 0; w < NW; ++w) tile_max = max(tile_max, s_max[w]);
+  const int todo_total = min((int)(c.range.y - c.range.x), (int)tile_max);
+  const int rounds = (todo_total + BATCH - 1) / BATCH;
 
   float T = T_final;
   uint32_t contributor = (uint32_t)todo_total;
@@ -150,12 +206,14 @@ __global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_bwd_kernel(RenderArgs 
   const unsigned row_lane = lane & 15;  // lane within its DPP row of 16
   for (int r = 0; r < rounds; ++r) {
     __syncthreads();
-    const int pos = r * DMGS_BLOCK_SIZE + threadIdx.x;
-    if (pos < todo_total) stage_splat(L, threadIdx.x, a.splats, a.point_list[range.x + todo_total - pos - 1], a.P);
+    for (int s = threadIdx.x; s < BATCH; s += NT) {
+      const int pos = r * BATCH + s;
+      if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P);
 #pragma unroll
-    for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[threadIdx.x][k] = 0.f;
+      for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
+    }
     __syncthreads();
-    const int n = min(DMGS_BLOCK_SIZE, todo_total - r * DMGS_BLOCK_SIZE);
+    const int n = min(BATCH, todo_total - r * BATCH);
     for (int j = 0; j < n; ++j) {
       --contributor;
       float v[DMGS_GACC_USED];
@@ -171,16 +229,16 @@ __global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_bwd_kernel(RenderArgs 
         const float alpha = fminf(0.99f, co.w * G);
         if (power <= 0.0f && alpha >= 1.0f / 255.0f) {
           contrib = true;
-          const float4 c = L.rgb_id[j];
+          const float4 col = L.rgb_id[j];
           T = T / (1.f - alpha);
           const float dchannel_dcolor = alpha * T;
           acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
           acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
           acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
-          lc0 = c.x;
-          lc1 = c.y;
-          lc2 = c.z;
-          float dL_dalpha = (c.x - acc0) * g0 + (c.y - acc1) * g1 + (c.z - acc2) * g2;
+          lc0 = col.x;
+          lc1 = col.y;
+          lc2 = col.z;
+          float dL_dalpha = (col.x - acc0) * g0 + (col.y - acc1) * g1 + (col.z - acc2) * g2;
           v[6] = dchannel_dcolor * g0;
           v[7] = dchannel_dcolor * g1;
           v[8] = dchannel_dcolor * g2;
@@ -211,8 +269,8 @@ __global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_bwd_kernel(RenderArgs 
       if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[j][row_lane], mine);
     }
     __syncthreads();
-    // flush the batch: consecutive lanes cover the 9 values of one Gaussian
-    for (int i = threadIdx.x; i < n * DMGS_GACC_USED; i += DMGS_BLOCK_SIZE) {
+    // flush the batch: consecutive lanes cover the 9 values of one splat
+    for (int i = threadIdx.x; i < n * DMGS_GACC_USED; i += NT) {
       const int slot = i / DMGS_GACC_USED, k = i - slot * DMGS_GACC_USED;
       const float val = s_acc[slot][k];
       if (val != 0.f) {
@@ -223,16 +281,36 @@ __global__ __launch_bounds__(DMGS_BLOCK_SIZE) void render_bwd_kernel(RenderArgs 
   }
 }
 
-void launch_render_fwd(const RenderArgs& a, hipStream_t s) {
-  const int tiles = a.grid_x * a.grid_y;
-  if (tiles <= 0) return;
-  hipLaunchKernelGGL(render_fwd_kernel, dim3(tiles), dim3(DMGS_BLOCK_SIZE), 0, s, a);
+template <int SUB, int BATCH>
+static void launch_fwd_t(const RenderArgs& a, hipStream_t s) {
+  constexpr int Q = DMGS_BLOCK_X / SUB;
+  hipLaunchKernelGGL((render_fwd_kernel<SUB, BATCH>), dim3(a.num_tiles * Q * Q), dim3(SUB * SUB), 0, s, a);
 }
 
-void launch_render_bwd(const RenderArgs& a, hipStream_t s) {
-  const int tiles = a.grid_x * a.grid_y;
-  if (tiles <= 0) return;
-  hipLaunchKernelGGL(render_bwd_kernel, dim3(tiles), dim3(DMGS_BLOCK_SIZE), 0, s, a);
+template <int SUB, int BATCH>
+static void launch_bwd_t(const RenderArgs& a, hipStream_t s) {
+  constexpr int Q = DMGS_BLOCK_X / SUB;
+  hipLaunchKernelGGL((render_bwd_kernel<SUB, BATCH>), dim3(a.num_tiles * Q * Q), dim3(SUB * SUB), 0, s, a);
+}
+
+void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {
+  if (a.num_tiles <= 0) return;
+  switch (variant) {
+    case kRender16x128: launch_fwd_t<16, 128>(a, s); break;
+    case kRender8x64: launch_fwd_t<8, 64>(a, s); break;
+    case kRender8x128: launch_fwd_t<8, 128>(a, s); break;
+    default: launch_fwd_t<16, 256>(a, s); break;
+  }
+}
+
+void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant) {
+  if (a.num_tiles <= 0) return;
+  switch (variant) {
+    case kRender16x128: launch_bwd_t<16, 128>(a, s); break;
+    case kRender8x64: launch_bwd_t<8, 64>(a, s); break;
+    case kRender8x128: launch_bwd_t<8, 128>(a, s); break;
+    default: launch_bwd_t<16, 256>(a, s); break;
+  }
 }
 
 }  // namespace dmgs
