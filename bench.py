@@ -1,18 +1,30 @@
-"""Benchmark: fwd+bwd iterations per second of the Gaussian rasterizer at
-1M Gaussians, 800x800, on one MI355X.
+"""Benchmark: fwd+bwd view-iterations per second of the Gaussian rasterizer at
+1M Gaussians, 800x800, on one MI355X — batched (all views in one captured
+HIP graph) beside the plain one-view-per-launch pipeline.
 
-Workload (default ``--workload single``): 1M free Gaussians in FP32 — means a
+Default workload (``--workload views``): 1M free Gaussians in FP32 — means a
 60/40 mixture of the unit-ball interior and a radius-3 shell, log-scales
-N(-3.5, 0.4), normalised N(0,1)^4 quaternions, sigmoid(N(0,1)) opacity, SH
-degree 3 (DC N(0, 0.5), rest N(0, 0.1)) — seen by one 800x800 pinhole camera
-(FoV 50 deg, distance 4, white background).  One step = drop-in ``render`` +
-0.8 L1 + 0.2 (1 - SSIM) against a seeded target (fused HIP loss kernels, same
-objective as utils/loss_utils.py) + ``backward`` into all six parameter
-tensors.  Inputs are resident in HBM before timing starts.
+N(-3.5, 0.4), N(0,1)^4 quaternions (normalised by the model), sigmoid(N(0,1))
+opacity, SH degree 3 (DC N(0, 0.5), rest N(0, 0.1)) — seen by 8 seeded
+cameras on a radius-4 sphere looking at the origin, 800x800 each, FoV 50 deg,
+white background.  One iteration = render one view + 0.8 L1 + 0.2 (1 - SSIM)
+against a seeded target + backward into all raw parameters.
 
-Prints ONE JSON line.  ``roofline`` is for the dominant kernel, timed with
-device events on the launching stream over the timed region; ``cpu_baseline``
-times the FP64 CPU oracle on a bounded sample of the same workload.
+* plain:   the drop-in ``render`` + fused loss + autograd backward, one view
+           per call (host read of the instance count every view);
+* batched: ``dmgs_amd.batched`` — every view in one launch sequence, summed
+           loss, captured once in a HIP graph and replayed.
+
+``--workload uneven``: 3M Gaussians (shell radius 5), 16 orbit cameras at
+radius 4 with resolutions drawn from {512^2, 800^2, 1280x720, 1600x900,
+1920x1080}.
+
+Prints ONE JSON line.  ``value`` is the batched throughput in view
+iterations per second; ``plain`` holds the one-view-per-launch figure on the
+same views.  ``roofline`` is for the dominant kernel of the batched step,
+timed with device events on the launching stream over eager steps of the
+same workload; ``cpu_baseline`` times the FP64 CPU oracle on a bounded
+sample of one 800x800 view.
 """
 import argparse
 import json
@@ -26,56 +38,60 @@ import torch
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "fwd+bwd it/s @1M Gaussians 800² vs plain; render p50/p99 ms; HBM GB/s"
+UNEVEN_SIZES = [(512, 512), (800, 800), (1280, 720), (1600, 900), (1920, 1080)]
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--gaussians", type=int, default=1_000_000)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("views", "uneven"), default="views")
+    ap.add_argument("--gaussians", type=int, default=None, help="default 1M (views) / 3M (uneven)")
+    ap.add_argument("--views", type=int, default=None, help="default 8 (views) / 16 (uneven)")
     ap.add_argument("--width", type=int, default=800)
     ap.add_argument("--height", type=int, default=800)
     ap.add_argument("--cpu-baseline-seconds", type=float, default=20.0,
                     help="time budget of the sampled CPU-oracle baseline (0 disables)")
-    ap.add_argument("--render-samples", type=int, default=100, help="forward-only renders for p50/p99")
+    ap.add_argument("--render-samples", type=int, default=50, help="forward-only renders for p50/p99")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain one-view-per-launch comparison")
     return ap.parse_args()
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 class StageTimer:
-    """Device events around every C-ABI stage, recorded on the launching (current) stream."""
+    """Device events around named stages, recorded on the current stream."""
 
     def __init__(self):
         self.events = {}
-        self.enabled = True
 
     def __call__(self, name):
         timer = self
 
         class _Ctx:
             def __enter__(self):
-                if timer.enabled:
-                    self.a = torch.cuda.Event(enable_timing=True)
-                    self.a.record()
+                self.a = torch.cuda.Event(enable_timing=True)
+                self.a.record()
                 return self
 
             def __exit__(self, *exc):
-                if timer.enabled:
-                    b = torch.cuda.Event(enable_timing=True)
-                    b.record()
-                    timer.events.setdefault(name, []).append((self.a, b))
+                b = torch.cuda.Event(enable_timing=True)
+                b.record()
+                timer.events.setdefault(name, []).append((self.a, b))
                 return False
         return _Ctx()
 
     def averages_ms(self):
         return {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in self.events.items()}
 
-    def reset(self):
-        self.events = {}
-
 
 def algorithmic_bytes(P, V, K, X, T):
-    """Compulsory HBM bytes per stage for one view (each tensor read or written once)."""
+    """Compulsory HBM bytes per stage (each tensor read or written once).
+    P Gaussian entries, V visible entries, K instances, X pixels, T tiles,
+    each summed over the views of one launch."""
     return {
         "geometry": 236 * P + 4 * P + 48 * V + 8 * P,
         "binning": 20 * V + 12 * K + 24 * K + 8 * K + 8 * T,
@@ -85,10 +101,83 @@ def algorithmic_bytes(P, V, K, X, T):
     }
 
 
-def cpu_baseline(model, cam, target, budget_s):
-    """FP64 CPU oracle on a bounded sample: full preprocess, binning and
-    geometry backward; blend forward+backward on random tiles until the budget
-    is spent, extrapolated by instance count.  Returns the baseline dict."""
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def build_workload(args, dev):
+    from dmgs_amd import cameras, scenes
+    if args.workload == "views":
+        N = args.gaussians or 1_000_000
+        B = args.views or 8
+        model = scenes.free_scene(N, seed=0, layout="mixture", shell_radius=3.0, device=dev)
+        pos = cameras.sphere_positions(B, 4.0, seed=2)
+        sizes = [(args.width, args.height)] * B
+        desc = (f"{N} free Gaussians (60/40 unit ball / r=3 shell), {B} orbit views (r=4) of "
+                f"{args.width}x{args.height}, fwd+bwd of 0.8 L1 + 0.2 (1-SSIM) per view")
+    else:
+        N = args.gaussians or 3_000_000
+        B = args.views or 16
+        model = scenes.free_scene(N, seed=0, layout="mixture", shell_radius=5.0, device=dev)
+        pos = cameras.sphere_positions(B, 4.0, seed=2)
+        g = np.random.default_rng(2)
+        sizes = [UNEVEN_SIZES[i] for i in g.integers(0, len(UNEVEN_SIZES), size=B)]
+        desc = (f"{N} free Gaussians (60/40 unit ball / r=5 shell), {B} orbit views (r=4), resolutions drawn "
+                f"from 512^2..1920x1080, fwd+bwd of summed 0.8 L1 + 0.2 (1-SSIM)")
+    cams = [cameras.camera_at(p, w, h, fov_deg=50.0, device=dev) for p, (w, h) in zip(pos, sizes)]
+    targets = [scenes.target_image(h, w, seed=1 + i, device=dev) for i, (w, h) in enumerate(sizes)]
+    return model, cams, targets, sizes, desc
+
+
+def run_plain(model, cams, targets, bg, steps, warmup):
+    """Drop-in render + fused loss + autograd backward, one view per call."""
+    from dmgs_amd import rasterizer as RZ
+    from dmgs_amd.gaussian_renderer import PipelineParams, render
+    from dmgs_amd.loss import fused_dssim_l1
+    model.requires_grad_(True)
+    pipe = PipelineParams()
+
+    def step():
+        for p in model.parameters():
+            p.grad = None
+        for cam, tgt in zip(cams, targets):
+            out = render(cam, model, pipe, bg)
+            fused_dssim_l1(out["render"], tgt, 0.2).backward()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    timer = StageTimer()
+    RZ.set_stage_hook(timer)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    RZ.set_stage_hook(None)
+    model.requires_grad_(False)
+    B = len(cams)
+    return {"value": round(B * steps / elapsed, 3), "unit": "it/s", "ms_per_view": round(1000 * elapsed / (B * steps), 4),
+            "stage_ms": {k: round(v, 4) for k, v in timer.averages_ms().items()}}
+
+
+def plain_latency(model, cam, bg, samples):
+    from dmgs_amd.gaussian_renderer import PipelineParams, render
+    pipe = PipelineParams()
+    lat = []
+    with torch.no_grad():
+        for _ in range(samples):
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            render(cam, model, pipe, bg)
+            torch.cuda.synchronize()
+            lat.append(1000.0 * (time.perf_counter() - a))
+    return lat
+
+
+def cpu_baseline(model, cam, budget_s):
+    """FP64 CPU oracle on a bounded sample of one view: full preprocess,
+    binning and geometry backward; blend forward+backward on random tiles
+    until the budget is spent, extrapolated by instance count."""
     from oracle import raster as OR
     threads = min(16, len(os.sched_getaffinity(0)))
     torch.set_num_threads(threads)
@@ -129,8 +218,8 @@ def cpu_baseline(model, cam, target, budget_s):
     t_gb = time.perf_counter() - a
     t_blend = t_tiles * K / max(done_k, 1)
     total = (t1 - t0) + (t2 - t1) + t_blend + t_gb
-    return {"value": 1.0 / total, "unit": "it/s", "cores": threads, "kind": "port",
-            "sample": (f"FP64 torch oracle, {P} Gaussians {W}x{H}: preprocess {t1 - t0:.2f}s, binning "
+    return {"value": round(1.0 / total, 6), "unit": "it/s", "cores": threads, "kind": "port",
+            "sample": (f"FP64 torch oracle, one {W}x{H} view of {P} Gaussians: preprocess {t1 - t0:.2f}s, binning "
                        f"{t2 - t1:.2f}s and geometry backward {t_gb:.2f}s over all Gaussians; blend fwd+bwd on "
                        f"{n_tiles} random tiles ({done_k} of {K} instances, {t_tiles:.1f}s) extrapolated by "
                        f"instance count to {t_blend:.1f}s; loss excluded")}
@@ -148,125 +237,6 @@ def _cpu_cam(cam):
     return c
 
 
-def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the rasterizer path here is "
-              "single-GPU and no multi-GPU figure is produced.", file=sys.stderr)
-        sys.exit(2)
-    from dmgs_amd import cameras, scenes
-    from dmgs_amd import rasterizer as RZ
-    from dmgs_amd.gaussian_renderer import PipelineParams, render
-    from dmgs_amd.loss import fused_dssim_l1
-
-    dev = "cuda"
-    torch.cuda.set_device(0)
-    W, H, N = args.width, args.height, args.gaussians
-    model = scenes.free_scene(N, seed=0, layout="mixture", shell_radius=3.0, device=dev).requires_grad_(True)
-    cam = cameras.front_camera(W, H, fov_deg=50.0, distance=4.0, device=dev)
-    target = scenes.target_image(H, W, seed=1, device=dev)
-    bg = torch.ones(3, device=dev)
-    pipe = PipelineParams()
-
-    def step():
-        for p in model.parameters():
-            p.grad = None
-        out = render(cam, model, pipe, bg)
-        loss = fused_dssim_l1(out["render"], target, 0.2)
-        loss.backward()
-        return out
-
-    log(f"scene ready: {N} Gaussians, {W}x{H}; warmup {args.warmup}")
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    log(f"timing {args.steps} steps")
-    timer = StageTimer()
-    RZ.set_stage_hook(timer)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    RZ.set_stage_hook(None)
-    stage_ms = timer.averages_ms()
-    ms_step = 1000.0 * elapsed / args.steps
-    it_s = args.steps / elapsed
-
-    # sizes of the workload (from the last step's outputs)
-    with torch.no_grad():
-        radii = out["radii"]
-        V = int((radii > 0).sum().item())
-        _, _, st = RZ.forward_raw(model.get_xyz.detach(), model.get_opacity.detach(), model.get_features.detach(),
-                                  None, model.get_scaling.detach(), model.get_rotation.detach(), None,
-                                  _settings(cam, bg))
-        K = st["K"]
-    X = W * H
-    T = ((W + 15) // 16) * ((H + 15) // 16)
-    ab = algorithmic_bytes(N, V, K, X, T)
-    kern = max(("render_forward", "render_backward"), key=lambda k: stage_ms.get(k, 0.0))
-    achieved = ab[kern] / (stage_ms[kern] * 1e-3) / 1e9
-    traffic = _read_traffic(kern)
-
-    # forward-only render latency (plain path)
-    lat = []
-    with torch.no_grad():
-        for _ in range(args.render_samples):
-            torch.cuda.synchronize()
-            a = time.perf_counter()
-            render(cam, model, pipe, bg)
-            torch.cuda.synchronize()
-            lat.append(1000.0 * (time.perf_counter() - a))
-    lat = np.array(lat)
-
-    total_bytes = sum(ab.values())
-    result = {
-        "metric": METRIC,
-        "value": round(it_s, 3),
-        "unit": "it/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{N} free Gaussians (60/40 unit ball / r=3 shell), {W}x{H}, one view, "
-                               "fwd+bwd of 0.8 L1 + 0.2 (1-SSIM)",
-                   "gaussians": N, "width": W, "height": H, "path": "plain", "visible": V, "instances": K},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kern,
-                     "kernel_ms": round(stage_ms[kern], 4),
-                     "algorithmic_bytes": ab[kern]},
-        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "render_ms_p50": round(float(np.percentile(lat, 50)), 3),
-        "render_ms_p99": round(float(np.percentile(lat, 99)), 3),
-        "step_algorithmic_GBs": round(total_bytes / (ms_step * 1e-3) / 1e9, 2),
-    }
-    log(f"{it_s:.2f} it/s; stages {stage_ms}")
-    if args.cpu_baseline_seconds > 0:
-        log("cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(model, cam, target, args.cpu_baseline_seconds)
-    else:
-        result["cpu_baseline"] = None
-    print(json.dumps(result), flush=True)
-
-
-def _settings(cam, bg):
-    from dmgs_amd.rasterizer import GaussianRasterizationSettings
-    return GaussianRasterizationSettings(
-        image_height=cam.image_height, image_width=cam.image_width, tanfovx=math.tan(cam.FoVx * 0.5),
-        tanfovy=math.tan(cam.FoVy * 0.5), bg=bg, scale_modifier=1.0, viewmatrix=cam.world_view_transform,
-        projmatrix=cam.full_proj_transform, sh_degree=3, campos=cam.camera_center, prefiltered=False, debug=False)
-
-
 def _read_traffic(kernel):
     """HBM bytes per launch of ``kernel`` measured with rocprofv3 PMC counters
     (committed under profiles/), or None when no measurement exists."""
@@ -276,6 +246,125 @@ def _read_traffic(kernel):
             return json.load(f).get(kernel)
     except (OSError, ValueError):
         return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the rasterizer path here is "
+              "single-GPU and no multi-GPU figure is produced.", file=sys.stderr)
+        sys.exit(2)
+    from dmgs_amd import batched as BT
+
+    dev = "cuda"
+    torch.cuda.set_device(0)
+    model, cams, targets, sizes, desc = build_workload(args, dev)
+    bg = torch.ones(3, device=dev)
+    B, N = len(cams), model.num
+    log(f"workload ready: {desc}")
+
+    # ---- batched: plan, autotune, capture, time graph replays ----
+    params = BT.pack_model(model)
+    r = BT.BatchedRenderer(cams, bg, N, sh_degree=3)
+    K_cap = r.plan(params)
+    loss = BT.BatchedLoss(r, targets, 0.2)
+    step = BT.BatchedStep(r, loss, params)
+    tune = r.autotune(params, reps=2)
+    log(f"planned K_cap={K_cap}; autotune ms {tune} -> variant {r.variant}")
+    step.capture()
+    for _ in range(args.warmup):
+        step.replay()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} batched steps ({B} views each)")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step.replay()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if r.overflowed():
+        raise RuntimeError("instance capacity overflowed during the timed region")
+    it_s = B * args.steps / elapsed
+    ms_step = 1000.0 * elapsed / args.steps
+    K = int(r.instance_count().item())
+    V = int((r.radii() > 0).sum().item())
+    X, T = r.total_pixels, r.total_tiles
+    losses = loss.losses[:, 0].sum().item()
+
+    # ---- stage timing of the same step, eager, with device events ----
+    timer = StageTimer()
+    for _ in range(max(3, args.steps // 2)):
+        with timer("geometry+binning"):
+            r.forward(params, step.images, stages=BT.STAGE_GEOMETRY)
+        with timer("render_forward"):
+            r.forward(params, step.images, stages=BT.STAGE_BLEND)
+        with timer("loss"):
+            loss.forward(step.images)
+            loss.backward(step.images, step.dimages)
+        with timer("render_backward"):
+            r.backward(params, step.dimages, step.grads, stages=BT.STAGE_BLEND)
+        with timer("geometry_backward"):
+            r.backward(params, step.dimages, step.grads, stages=BT.STAGE_GEOMETRY)
+    torch.cuda.synchronize()
+    stage_ms = timer.averages_ms()
+    ab = algorithmic_bytes(B * N, V, K, X, T)
+    kern = max(("render_forward", "render_backward"), key=lambda k: stage_ms[k])
+    achieved = ab[kern] / (stage_ms[kern] * 1e-3) / 1e9
+    step_bytes = ab["geometry"] + ab["binning"] + ab["render_forward"] + ab["render_backward"] + \
+        ab["geometry_backward"]
+
+    # ---- forward-only latency: batch, and the plain single view ----
+    blat = []
+    for _ in range(max(5, args.render_samples // 5)):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        r.forward(params, step.images)
+        torch.cuda.synchronize()
+        blat.append(1000.0 * (time.perf_counter() - a))
+    plat = plain_latency(model, cams[0], bg, args.render_samples)
+
+    plain = None
+    if not args.no_plain:
+        log("timing the plain one-view-per-launch path on the same views")
+        plain = run_plain(model, cams, targets, bg, args.steps, args.warmup)
+
+    result = {
+        "metric": METRIC,
+        "value": round(it_s, 3),
+        "unit": "it/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "gaussians": N, "views": B, "sizes": [f"{w}x{h}" for w, h in sizes],
+                   "path": "batched",
+                   "step": f"{B} views fwd+bwd in one HIP graph replay; value = views/s",
+                   "visible_entries": V, "instances": K, "instance_capacity": K_cap, "render_variant": r.variant,
+                   "summed_loss": round(losses, 6)},
+        "plain": plain,
+        "vs_plain": round(it_s / plain["value"], 3) if plain else None,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": _read_traffic(f"batched_{kern}"),
+                     "kernel": kern, "kernel_ms": round(stage_ms[kern], 4), "algorithmic_bytes": ab[kern]},
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "autotune_ms": {str(k): round(v, 4) for k, v in tune.items()},
+        "render_ms_p50": round(float(np.percentile(plat, 50)), 3),
+        "render_ms_p99": round(float(np.percentile(plat, 99)), 3),
+        "batch_render_ms_p50": round(float(np.percentile(blat, 50)), 3),
+        "batch_render_ms_p99": round(float(np.percentile(blat, 99)), 3),
+        "step_algorithmic_GBs": round(step_bytes / (ms_step * 1e-3) / 1e9, 2),
+    }
+    log(f"batched {it_s:.2f} it/s; plain {plain['value'] if plain else 'skipped'}; stages {stage_ms}")
+    if args.cpu_baseline_seconds > 0:
+        log("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(model, cams[0], args.cpu_baseline_seconds)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

@@ -25,6 +25,16 @@ class DmgsSettings(ctypes.Structure):
     ]
 
 
+class DmgsView(ctypes.Structure):
+    """struct dmgs_view (include/dmgs.h): one camera of a batched launch."""
+    _fields_ = [
+        ("viewmatrix", c_float * 16), ("projmatrix", c_float * 16), ("campos", c_float * 4), ("bg", c_float * 4),
+        ("tanfovx", c_float), ("tanfovy", c_float), ("focal_x", c_float), ("focal_y", c_float),
+        ("width", c_int32), ("height", c_int32), ("grid_x", c_int32), ("grid_y", c_int32),
+        ("tile_base", c_int32), ("reserved", c_int32), ("pixel_base", c_int64),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "dmgs_last_error": (ctypes.c_char_p, []),
@@ -61,6 +71,16 @@ _SIGNATURES = {
                                          c_void_p, c_void_p, c_void_p]),
     "dmgs_scan_scratch_size": (c_size_t, [c_int64]),
     "dmgs_inclusive_scan_u32": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "dmgs_batched_prepare_views": (c_int32, [ctypes.POINTER(DmgsView), c_int32, ctypes.POINTER(c_int32),
+                                             ctypes.POINTER(c_int64)]),
+    "dmgs_batched_tile_view": (c_int32, [ctypes.POINTER(DmgsView), c_int32, c_void_p, c_int32]),
+    "dmgs_batched_workspace_size": (c_size_t, [c_int32, c_int64, c_int64, c_int32, c_int64]),
+    "dmgs_batched_offsets": (c_int32, [c_int32, c_int64, c_int64, c_int32, c_int64, ctypes.POINTER(c_size_t)]),
+    "dmgs_batched_forward": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_float, c_void_p, c_int64, c_int32,
+                                       c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p]),
+    "dmgs_batched_backward": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_float, c_void_p, c_int64, c_int32,
+                                        c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
+                                        c_void_p]),
 }
 
 _lib = None

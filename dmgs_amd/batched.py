@@ -1,0 +1,324 @@
+"""Batched multi-view rendering: B views of one scene in one launch sequence.
+
+This replaces the per-camera loop of the training and evaluation scripts
+(train_geo_stage3.py:104-124 renders one camera per iteration through
+``gaussian_renderer.render`` and reads ``num_rendered`` back to the host on
+every call).  Here:
+
+* the scene is one component-major tensor ``params`` of 59 x P raw
+  parameters (``pack_params``); the exp / normalise / sigmoid activations of
+  scene/gaussian_model.py:95-118 run inside the kernels, and the backward
+  returns raw-parameter gradients in the same layout (``unpack_grads``);
+* all views go through one preprocess, one depth sort, one duplicate, one
+  tile sort and one blend launch (views may differ in resolution);
+* the instance count stays on the device: the list capacity is planned once
+  from a counting pass and checked after the fact (``overflowed``), so a
+  steady-state step has no host synchronisation and can be captured in a
+  HIP graph (``BatchedStep``);
+* the blend work-group shape / LDS batch depth is picked by ``autotune``.
+
+Images and gradients equal those of B single-view ``render`` calls followed
+by autograd through the activations (same per-view arithmetic).
+"""
+import ctypes
+import math
+import time
+
+import torch
+
+from . import _native as N
+
+CHANNELS = 59
+XYZ, SCALE, ROT, OPACITY, SH = 0, 3, 6, 10, 11
+STAGE_GEOMETRY, STAGE_BLEND, STAGES_ALL = 1, 2, 3
+RENDER_VARIANTS = ("16x16 px, 256-deep LDS batch", "16x16 px, 128-deep", "8x8 px, 64-deep", "8x8 px, 128-deep")
+
+
+def pack_params(xyz, log_scale, rotation, opacity_logit, features_dc, features_rest, out=None):
+    """Raw model parameters -> (59, P) component-major float32 tensor.
+
+    Channels: 0-2 xyz, 3-5 log-scale, 6-9 raw quaternion (w, x, y, z),
+    10 opacity logit, 11 + 3k + c SH coefficient k of colour c (k = 0 is the
+    DC term; degrees below 3 are zero-padded to 16 coefficients)."""
+    P = xyz.shape[0]
+    if out is None:
+        out = torch.zeros(CHANNELS, P, dtype=torch.float32, device=xyz.device)
+    with torch.no_grad():
+        out[XYZ:XYZ + 3] = xyz.detach().t()
+        out[SCALE:SCALE + 3] = log_scale.detach().t()
+        out[ROT:ROT + 4] = rotation.detach().t()
+        out[OPACITY] = opacity_logit.detach().reshape(P)
+        sh = torch.cat((features_dc.detach(), features_rest.detach()), dim=1).reshape(P, -1)
+        out[SH:SH + sh.shape[1]] = sh.t()
+        out[SH + sh.shape[1]:] = 0
+    return out
+
+
+def pack_model(model, out=None):
+    """``pack_params`` of a model with the reference's raw attributes."""
+    return pack_params(model._xyz, model._scaling, model._rotation, model._opacity, model._features_dc,
+                       model._features_rest, out=out)
+
+
+def unpack_grads(grads, sh_coeffs=16):
+    """(59, P) gradients -> dict shaped like the model's raw parameters."""
+    P = grads.shape[1]
+    sh = grads[SH:SH + 3 * sh_coeffs].t().reshape(P, sh_coeffs, 3)
+    return {
+        "_xyz": grads[XYZ:XYZ + 3].t(),
+        "_scaling": grads[SCALE:SCALE + 3].t(),
+        "_rotation": grads[ROT:ROT + 4].t(),
+        "_opacity": grads[OPACITY].reshape(P, 1),
+        "_features_dc": sh[:, :1],
+        "_features_rest": sh[:, 1:],
+    }
+
+
+def _view_record(cam, bg):
+    v = N.DmgsView()
+    wv = cam.world_view_transform.detach().float().cpu().contiguous().reshape(-1).tolist()
+    fp = cam.full_proj_transform.detach().float().cpu().contiguous().reshape(-1).tolist()
+    cc = cam.camera_center.detach().float().cpu().reshape(-1).tolist()
+    b = torch.as_tensor(bg).detach().float().cpu().reshape(-1).tolist()
+    for i in range(16):
+        v.viewmatrix[i] = wv[i]
+        v.projmatrix[i] = fp[i]
+    for i in range(3):
+        v.campos[i] = cc[i]
+        v.bg[i] = b[i]
+    v.tanfovx = math.tan(cam.FoVx * 0.5)
+    v.tanfovy = math.tan(cam.FoVy * 0.5)
+    v.width, v.height = int(cam.image_width), int(cam.image_height)
+    return v
+
+
+class BatchedRenderer:
+    """Forward / backward of B views of a P-Gaussian scene.
+
+    ``cameras`` carry the reference Camera fields the renderer reads
+    (world_view_transform, full_proj_transform, camera_center, FoVx, FoVy,
+    image_width, image_height); ``bg`` is one colour or one per view."""
+
+    def __init__(self, cameras, bg, num_gaussians, sh_degree=3, scale_modifier=1.0, device="cuda",
+                 headroom=1.2, variant=0):
+        if not cameras:
+            raise ValueError("BatchedRenderer needs at least one camera")
+        self.lib = N.load()
+        self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.B, self.P = len(cameras), int(num_gaussians)
+        self.sh_degree, self.scale_modifier = int(sh_degree), float(scale_modifier)
+        self.headroom, self.variant = float(headroom), int(variant)
+        bgs = bg if isinstance(bg, (list, tuple)) else [bg] * self.B
+        self.views = (N.DmgsView * self.B)(*[_view_record(c, b) for c, b in zip(cameras, bgs)])
+        tt, tp = N.c_int32(0), N.c_int64(0)
+        N.check(self.lib.dmgs_batched_prepare_views(self.views, self.B, ctypes.byref(tt), ctypes.byref(tp)),
+                "dmgs_batched_prepare_views")
+        self.total_tiles, self.total_pixels = tt.value, tp.value
+        tv = torch.empty(self.total_tiles, dtype=torch.int32)
+        N.check(self.lib.dmgs_batched_tile_view(self.views, self.B, ctypes.c_void_p(tv.data_ptr()),
+                                                self.total_tiles), "dmgs_batched_tile_view")
+        self.tile_view = tv.to(self.device)
+        raw = bytes(memoryview(self.views).cast("B"))
+        self.views_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
+        self.sizes = [(v.height, v.width) for v in self.views]
+        self.pixel_base = [v.pixel_base for v in self.views]
+        self.K_cap = 0
+        self.ws = None
+        self._offsets = None
+
+    # ---- planning ---------------------------------------------------------
+    def _alloc(self, K_cap):
+        n = self.lib.dmgs_batched_workspace_size(self.B, self.P, K_cap, self.total_tiles, self.total_pixels)
+        if n == 0:
+            raise RuntimeError("dmgs_batched_workspace_size rejected the batch")
+        offs = (N.c_size_t * 6)()
+        N.check(self.lib.dmgs_batched_offsets(self.B, self.P, K_cap, self.total_tiles, self.total_pixels, offs),
+                "dmgs_batched_offsets")
+        return torch.empty(n, dtype=torch.uint8, device=self.device), list(offs)
+
+    def count_instances(self, params):
+        """Run preprocess + depth sort + scan only and return the number of
+        (Gaussian, tile) instances of the batch (one host read)."""
+        ws, offs = (self.ws, self._offsets) if self.ws is not None else self._alloc(0)
+        self._call_forward(params, 0, ws, None)
+        return int(ws[offs[1]:offs[1] + 4].view(torch.int32).item()) & 0xFFFFFFFF
+
+    def plan(self, params, K_cap=None):
+        """Size the instance lists for ``params`` (with headroom) and allocate
+        the workspace.  Returns the capacity."""
+        if K_cap is None:
+            K = self.count_instances(params)
+            K_cap = max(1024, int(math.ceil(K * self.headroom)))
+        self.K_cap = int(K_cap)
+        self.ws, self._offsets = self._alloc(self.K_cap)
+        return self.K_cap
+
+    def _require_plan(self):
+        if self.ws is None:
+            raise RuntimeError("BatchedRenderer.plan() must run before forward/backward")
+
+    def _check_params(self, params):
+        if params.dtype != torch.float32 or not params.is_contiguous() or tuple(params.shape) != (CHANNELS, self.P):
+            raise ValueError(f"params must be a contiguous float32 ({CHANNELS}, {self.P}) tensor")
+        if params.device != self.device:
+            raise ValueError("params must live on the renderer's device")
+
+    # ---- launches ---------------------------------------------------------
+    def _call_forward(self, params, K_cap, ws, images, stages=STAGES_ALL):
+        self._check_params(params)
+        N.check(self.lib.dmgs_batched_forward(
+            N.ptr(self.views_dev), self.B, self.P, self.sh_degree, self.scale_modifier, N.ptr(params), K_cap,
+            self.total_tiles, self.total_pixels, N.ptr(self.tile_view), N.ptr(ws), N.ptr(images), self.variant,
+            stages, N.stream_handle(self.device)), "dmgs_batched_forward")
+
+    def new_images(self):
+        return torch.empty(3 * self.total_pixels, dtype=torch.float32, device=self.device)
+
+    def forward(self, params, images=None, stages=STAGES_ALL):
+        """Render every view into one packed buffer (view v at 3*pixel_base[v],
+        laid out (3, H, W)); returns it.  ``stages`` may split the call into
+        the per-Gaussian part and the blend (run in that order)."""
+        self._require_plan()
+        if images is None:
+            images = self.new_images()
+        self._call_forward(params, self.K_cap, self.ws, images, stages)
+        return images
+
+    def backward(self, params, dL_dimages, grads=None, stages=STAGES_ALL):
+        """Raw-parameter gradients (59, P) of sum over views of <dL/dimage, image>
+        (``stages``: blend backward, then the per-Gaussian backward)."""
+        self._require_plan()
+        self._check_params(params)
+        if dL_dimages.numel() != 3 * self.total_pixels or not dL_dimages.is_contiguous():
+            raise ValueError("dL_dimages must be a contiguous buffer packed like the images")
+        if grads is None:
+            grads = torch.empty(CHANNELS, self.P, dtype=torch.float32, device=self.device)
+        N.check(self.lib.dmgs_batched_backward(
+            N.ptr(self.views_dev), self.B, self.P, self.sh_degree, self.scale_modifier, N.ptr(params), self.K_cap,
+            self.total_tiles, self.total_pixels, N.ptr(self.tile_view), N.ptr(self.ws), N.ptr(dL_dimages),
+            N.ptr(grads), self.variant, stages, N.stream_handle(self.device)), "dmgs_batched_backward")
+        return grads
+
+    # ---- results ----------------------------------------------------------
+    def image(self, packed, v):
+        """(3, H, W) view of image ``v`` inside a packed buffer."""
+        H, W = self.sizes[v]
+        o = 3 * self.pixel_base[v]
+        return packed[o:o + 3 * H * W].view(3, H, W)
+
+    def instance_count(self):
+        """Device tensor (1,) holding the instance count of the last forward."""
+        self._require_plan()
+        o = self._offsets[1]
+        return self.ws[o:o + 4].view(torch.int32)
+
+    def overflowed(self):
+        """True when the last forward needed more instances than planned
+        (its lists were truncated; re-plan and render again).  Reads back."""
+        return (int(self.instance_count().item()) & 0xFFFFFFFF) > self.K_cap
+
+    def radii(self):
+        """(B, P) int32 radii of the last forward."""
+        self._require_plan()
+        o = self._offsets[0]
+        return self.ws[o:o + 4 * self.B * self.P].view(torch.int32).view(self.B, self.P)
+
+    # ---- autotuning -------------------------------------------------------
+    def autotune(self, params, dL_dimages=None, reps=3):
+        """Time forward + backward for every blend variant and keep the
+        fastest.  Returns {variant: ms}."""
+        self._require_plan()
+        if dL_dimages is None:
+            dL_dimages = torch.full((3 * self.total_pixels,), 1e-3, device=self.device)
+        images = self.new_images()
+        grads = torch.empty(CHANNELS, self.P, dtype=torch.float32, device=self.device)
+        times = {}
+        for v in range(len(RENDER_VARIANTS)):
+            self.variant = v
+            self.forward(params, images)
+            self.backward(params, dL_dimages, grads)
+            torch.cuda.synchronize(self.device)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                self.forward(params, images)
+                self.backward(params, dL_dimages, grads)
+            torch.cuda.synchronize(self.device)
+            times[v] = 1000.0 * (time.perf_counter() - t0) / reps
+        self.variant = min(times, key=times.get)
+        return times
+
+
+class BatchedLoss:
+    """Summed per-view (1 - lambda) L1 + lambda (1 - SSIM) against fixed
+    targets (train_geo_stage3.py:122-124 applied to every view of the
+    batch), on the fused loss kernels; no host synchronisation."""
+
+    def __init__(self, renderer, targets, lambda_dssim=0.2):
+        if len(targets) != renderer.B:
+            raise ValueError("one target per view")
+        self.r, self.lam = renderer, float(lambda_dssim)
+        self.lib = N.load()
+        dev = renderer.device
+        self.targets = renderer.new_images()
+        for v, t in enumerate(targets):
+            renderer.image(self.targets, v).copy_(t.to(device=dev, dtype=torch.float32))
+        self.ws = [torch.empty(self.lib.dmgs_dssim_l1_workspace_size(3, h, w), dtype=torch.uint8, device=dev)
+                   for h, w in renderer.sizes]
+        self.losses = torch.zeros(renderer.B, 3, dtype=torch.float32, device=dev)
+        self.one = torch.ones(1, dtype=torch.float32, device=dev)
+
+    def forward(self, images):
+        r, s = self.r, N.stream_handle(self.r.device)
+        for v, (h, w) in enumerate(r.sizes):
+            N.check(self.lib.dmgs_dssim_l1_forward(N.ptr(r.image(images, v)), N.ptr(r.image(self.targets, v)), 3,
+                                                   h, w, self.lam, N.ptr(self.ws[v]), N.ptr(self.losses[v]), s),
+                    "dmgs_dssim_l1_forward")
+        return self.losses
+
+    def backward(self, images, dimages):
+        r, s = self.r, N.stream_handle(self.r.device)
+        for v, (h, w) in enumerate(r.sizes):
+            N.check(self.lib.dmgs_dssim_l1_backward(N.ptr(r.image(images, v)), N.ptr(r.image(self.targets, v)), 3,
+                                                    h, w, self.lam, N.ptr(self.ws[v]), N.ptr(self.one),
+                                                    N.ptr(r.image(dimages, v)), s), "dmgs_dssim_l1_backward")
+        return dimages
+
+
+class BatchedStep:
+    """One fwd+bwd training step over all views: batched render, summed loss,
+    loss backward, batched render backward into raw-parameter gradients.
+
+    ``run()`` launches eagerly; ``capture()`` records the same launches in a
+    HIP graph and ``replay()`` re-issues them (the scene tensor, targets and
+    views are fixed at capture; update ``params`` in place between replays)."""
+
+    def __init__(self, renderer, loss, params):
+        self.r, self.loss, self.params = renderer, loss, params
+        self.images = renderer.new_images()
+        self.dimages = renderer.new_images()
+        self.grads = torch.empty(CHANNELS, renderer.P, dtype=torch.float32, device=renderer.device)
+        self.graph = None
+
+    def run(self):
+        self.r.forward(self.params, self.images)
+        self.loss.forward(self.images)
+        self.loss.backward(self.images, self.dimages)
+        self.r.backward(self.params, self.dimages, self.grads)
+        return self.loss.losses
+
+    def capture(self):
+        self.run()  # warm the kernels and the allocator outside capture
+        torch.cuda.synchronize(self.r.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        self.graph = g
+        return g
+
+    def replay(self):
+        if self.graph is None:
+            raise RuntimeError("capture() first")
+        self.graph.replay()
+        return self.loss.losses

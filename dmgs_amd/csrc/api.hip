@@ -135,6 +135,11 @@ void fill_geom_common(const dmgs_settings* s, float& fx, float& fy, int& gx, int
 
 }  // namespace
 
+namespace dmgs {
+// Error reporting for the entry points defined in other translation units.
+void set_last_error(const char* msg) { g_last_error = msg; }
+}  // namespace dmgs
+
 extern "C" {
 
 const char* dmgs_last_error(void) { return g_last_error.c_str(); }
@@ -243,9 +248,18 @@ int dmgs_forward_binning(const dmgs_settings* s, int64_t P, int64_t K, const voi
     BinLayout BL((size_t)K);
     uint32_t* ka = at<uint32_t>(binning_ws, BL.keys_a);
     uint32_t* va = at<uint32_t>(binning_ws, BL.vals_a);
-    dmgs::launch_duplicate((int)P, at<uint32_t>(geom_ws, GL.order_a), at<uint32_t>(geom_ws, GL.offsets),
-                           at<float4>(geom_ws, GL.splats), gx, gy, ka, va, (uint32_t)K,
-                           at<uint32_t>(binning_ws, BL.scratch), st);
+    dmgs::DupArgs d{};
+    d.P = (int)P;
+    d.order = at<uint32_t>(geom_ws, GL.order_a);
+    d.offsets = at<uint32_t>(geom_ws, GL.offsets);
+    d.splats = at<float4>(geom_ws, GL.splats);
+    d.grid_x = gx;
+    d.grid_y = gy;
+    d.tile_keys = ka;
+    d.values = va;
+    d.K = (uint32_t)K;
+    d.chunk_first = at<uint32_t>(binning_ws, BL.scratch);
+    dmgs::launch_duplicate(d, st);
     if ((rc = check_launch("duplicate", st, s->debug))) return rc;
     dmgs::radix_sort_pairs(ka, va, at<uint32_t>(binning_ws, BL.keys_b), at<uint32_t>(binning_ws, BL.vals_b),
                            (size_t)K, tile_bits(T), at<uint32_t>(binning_ws, BL.scratch), st);
@@ -496,10 +510,9 @@ int dmgs_dssim_l1_forward(const float* img, const float* gt, int32_t C, int32_t 
   SsimLayout L(C, H, W);
   float w[11];
   ssim_window(w);
+  // the final reduction writes (loss, l1, ssim) straight to loss_out (no copy node when captured)
   dmgs::launch_dssim_forward(img, gt, C, H, W, w, lambda_dssim, at<float>(ws, L.coef), at<float>(ws, L.partials),
-                             at<float>(ws, L.out), st);
-  if (hipMemcpyAsync(loss_out, at<float>(ws, L.out), 3 * sizeof(float), hipMemcpyDeviceToDevice, st) != hipSuccess)
-    return fail(DMGS_EHIP, "loss copy failed");
+                             loss_out, st);
   return check_launch("dssim forward", st, false);
 }
 

@@ -1,0 +1,174 @@
+// Multi-view forward/backward of the per-Gaussian stages for gfx950.
+//
+// A batch renders B views of one scene.  The scene is held component-major
+// (59 channels x P, see DMGS_MV_* below): each thread owns one Gaussian, reads
+// its raw parameters once with fully coalesced 4-byte lanes, applies the
+// model activations of scene/gaussian_model.py:26-41,95-118 (exp scale,
+// normalised quaternion, sigmoid opacity, SH = [dc, rest]) and produces the
+// splat records of all B views.  The backward walks the same B views,
+// accumulates the gradient of every view in registers (3D covariance, SH and
+// opacity gradients are view-independent sums) and writes the raw-parameter
+// gradients once, through the activation derivatives.
+//
+// Per-view math is the single-view kernels' (gs_math.h), so a batch of B
+// views gives the same images and the same summed gradients as B single-view
+// launches followed by autograd through the activations.
+#include "dmgs_common.h"
+#include "dmgs_kernels.h"
+#include "gs_math.h"
+
+namespace dmgs {
+
+// Activations with the accurate expf and IEEE division, as torch evaluates
+// exp / sigmoid / normalize in FP32, so the activated values match the
+// single-view path fed by the model getters.
+__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+struct MVGaussian {
+  float3 p, logs, s;  // position, log-scale, modifier * exp(log-scale)
+  float4 rq, q;       // raw and normalised quaternion
+  float rq_inv;       // 1 / max(|rq|, 1e-12)
+  float ol, o;        // opacity logit and sigmoid
+  float R[9];
+  float c3[6];
+  float sh[48];
+};
+
+__device__ __forceinline__ void load_mv(const MVArgs& a, int g, MVGaussian& G) {
+  const size_t P = (size_t)a.P;
+  const float* pr = a.params;
+  auto ch = [&](int c) { return pr[(size_t)c * P + g]; };
+  G.p = make_float3(ch(DMGS_MV_XYZ), ch(DMGS_MV_XYZ + 1), ch(DMGS_MV_XYZ + 2));
+  G.logs = make_float3(ch(DMGS_MV_SCALE), ch(DMGS_MV_SCALE + 1), ch(DMGS_MV_SCALE + 2));
+  const float m = a.scale_modifier;
+  G.s = make_float3(m * expf(G.logs.x), m * expf(G.logs.y), m * expf(G.logs.z));
+  G.rq = make_float4(ch(DMGS_MV_ROT), ch(DMGS_MV_ROT + 1), ch(DMGS_MV_ROT + 2), ch(DMGS_MV_ROT + 3));
+  const float nrm = fmaxf(sqrtf(G.rq.x * G.rq.x + G.rq.y * G.rq.y + G.rq.z * G.rq.z + G.rq.w * G.rq.w), 1e-12f);
+  G.rq_inv = 1.0f / nrm;
+  G.q = make_float4(G.rq.x / nrm, G.rq.y / nrm, G.rq.z / nrm, G.rq.w / nrm);
+  G.ol = ch(DMGS_MV_OPACITY);
+  G.o = sigmoidf(G.ol);
+#pragma unroll
+  for (int k = 0; k < 48; ++k) G.sh[k] = ch(DMGS_MV_SH + k);
+  quat_to_R(G.q, G.R);
+  cov3d_from_sR(G.s, G.R, G.c3);
+}
+
+__global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.P) return;
+  MVGaussian G;
+  load_mv(a, g, G);
+  for (int v = 0; v < a.B; ++v) {
+    const ViewDesc& vd = a.views[v];
+    const ViewParams vp = view_params(vd);
+    const size_t i = (size_t)v * a.P + g;
+    a.order[i] = (uint32_t)i;
+    SplatGeom o;
+    if (!project_splat(G.p, G.c3, vp, o)) {
+      a.radii[i] = 0;
+      a.tiles[i] = 0;
+      a.depth_keys[i] = 0xFFFFFFFFu;
+      continue;
+    }
+    float3 dir = make_float3(G.p.x - vd.campos[0], G.p.y - vd.campos[1], G.p.z - vd.campos[2]);
+    const float inv_len = 1.0f / sqrtf(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
+    dir = make_float3(dir.x * inv_len, dir.y * inv_len, dir.z * inv_len);
+    float3 rgb = sh_to_rgb(a.sh_degree, G.sh, dir);
+    rgb = make_float3(fmaxf(rgb.x, 0.0f), fmaxf(rgb.y, 0.0f), fmaxf(rgb.z, 0.0f));
+    float4* sp = a.splats + (size_t)DMGS_SPLAT_F4 * i;
+    sp[0] = make_float4(o.pix.x, o.pix.y, o.conic.x, o.conic.y);
+    sp[1] = make_float4(o.conic.z, G.o, rgb.x, rgb.y);
+    sp[2] = make_float4(rgb.z, o.depth, __int_as_float(o.radius), 0.0f);
+    a.radii[i] = o.radius;
+    a.tiles[i] = (uint32_t)o.tiles;
+    a.depth_keys[i] = __float_as_uint(o.depth);
+    if (a.gacc) {  // clear the backward accumulator of every visible (view, Gaussian)
+      float4* gz = reinterpret_cast<float4*>(a.gacc + i * DMGS_GACC_STRIDE);
+      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+      gz[0] = z;
+      gz[1] = z;
+      gz[2] = z;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.P) return;
+  MVGaussian G;
+  load_mv(a, g, G);
+  float3 dmean = make_float3(0.f, 0.f, 0.f);
+  float dc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  float dsh[48];
+#pragma unroll
+  for (int k = 0; k < 48; ++k) dsh[k] = 0.f;
+  float dop = 0.f;
+  bool any = false;
+  for (int v = 0; v < a.B; ++v) {
+    const size_t i = (size_t)v * a.P + g;
+    if (a.radii[i] <= 0) continue;
+    any = true;
+    const ViewDesc& vd = a.views[v];
+    const ViewParams vp = view_params(vd);
+    const float* gr = a.gacc + i * DMGS_GACC_STRIDE;
+    const float4 g03 = *reinterpret_cast<const float4*>(gr);
+    const float4 g47 = *reinterpret_cast<const float4*>(gr + 4);
+    const float g8 = gr[8];
+    const float3 dm = ewa_backward(G.p, G.c3, vp, g03.x, g03.y, g03.z, g03.w, g47.x, dc);
+    float b[16];
+    float3 gm;
+    const float3 dd = sh_backward(a.sh_degree, G.sh, G.p, vd.campos, make_float3(g47.z, g47.w, g8), b, gm);
+    dmean.x += dm.x + dd.x;
+    dmean.y += dm.y + dd.y;
+    dmean.z += dm.z + dd.z;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      dsh[3 * k] += b[k] * gm.x;
+      dsh[3 * k + 1] += b[k] * gm.y;
+      dsh[3 * k + 2] += b[k] * gm.z;
+    }
+    dop += g47.y;
+  }
+  float3 dlogs = make_float3(0.f, 0.f, 0.f);
+  float4 drq = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (any) {
+    float3 ds;
+    float4 dq;
+    cov3d_backward(dc, G.s, G.R, G.q, ds, dq);
+    // scale = exp(log-scale): d/dlog = dL/dscale * exp(log) (dL/dscale is taken w.r.t. mod * scale)
+    dlogs = make_float3(ds.x * expf(G.logs.x), ds.y * expf(G.logs.y), ds.z * expf(G.logs.z));
+    // q = rq / |rq|: d/drq = (dq - q (q . dq)) / |rq|
+    const float qd = G.q.x * dq.x + G.q.y * dq.y + G.q.z * dq.z + G.q.w * dq.w;
+    drq = make_float4((dq.x - G.q.x * qd) * G.rq_inv, (dq.y - G.q.y * qd) * G.rq_inv,
+                      (dq.z - G.q.z * qd) * G.rq_inv, (dq.w - G.q.w * qd) * G.rq_inv);
+  }
+  const size_t P = (size_t)a.P;
+  float* gd = a.grads;
+  auto put = [&](int c, float val) { gd[(size_t)c * P + g] = val; };
+  put(DMGS_MV_XYZ, dmean.x);
+  put(DMGS_MV_XYZ + 1, dmean.y);
+  put(DMGS_MV_XYZ + 2, dmean.z);
+  put(DMGS_MV_SCALE, dlogs.x);
+  put(DMGS_MV_SCALE + 1, dlogs.y);
+  put(DMGS_MV_SCALE + 2, dlogs.z);
+  put(DMGS_MV_ROT, drq.x);
+  put(DMGS_MV_ROT + 1, drq.y);
+  put(DMGS_MV_ROT + 2, drq.z);
+  put(DMGS_MV_ROT + 3, drq.w);
+  put(DMGS_MV_OPACITY, dop * G.o * (1.0f - G.o));
+#pragma unroll
+  for (int k = 0; k < 48; ++k) put(DMGS_MV_SH + k, dsh[k]);
+}
+
+void launch_mv_preprocess(const MVArgs& a, hipStream_t s) {
+  if (a.P <= 0 || a.B <= 0) return;
+  hipLaunchKernelGGL(mv_preprocess_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s) {
+  if (a.P <= 0 || a.B <= 0) return;
+  hipLaunchKernelGGL(mv_geometry_bwd_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+}
+
+}  // namespace dmgs

@@ -44,20 +44,59 @@ void launch_inclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t
 // at most 8 bits.  Data starts in (k0, v0); (k1, v1) is the ping-pong buffer.
 // The number of passes is always even, so the result ends in (k0, v0).
 size_t radix_scratch_words(size_t n);
+// n_dev (optional) holds the actual length on the device; n is then the capacity.
 void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
-                      uint32_t* scratch, hipStream_t s);
+                      uint32_t* scratch, hipStream_t s, const uint32_t* n_dev = nullptr);
 
 // tiles_touched gathered into depth order, ready for the scan.
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int n, hipStream_t s);
 
-// Emit (tile, gaussian) pairs in depth order.
-// The splat record carries the integer radius (0 = culled) in s2.z.
-// scratch holds duplicate_scratch_words(K) words.
+// Emit (tile, splat) pairs in depth order.  Single view: grid_x/grid_y give
+// the tile grid.  Several views: entry g belongs to view g / per_view and
+// tiles are numbered globally from that view's tile_base.
+struct DupArgs {
+  int P;                        // depth-sorted entries
+  const uint32_t* order;
+  const uint32_t* offsets;      // inclusive prefix of tiles touched, in depth order
+  const float4* splats;         // radius in s2.z
+  int grid_x, grid_y;
+  const ViewDesc* views;
+  uint32_t per_view;
+  uint32_t* tile_keys;
+  uint32_t* values;
+  uint32_t K;                   // capacity of tile_keys / values
+  const uint32_t* K_dev;        // optional device count (clamped to K)
+  uint32_t* chunk_first;        // duplicate_scratch_words(K) words
+};
 size_t duplicate_scratch_words(size_t K);
-void launch_duplicate(int P, const uint32_t* order, const uint32_t* offsets, const float4* splats, int grid_x,
-                      int grid_y, uint32_t* tile_keys, uint32_t* values, uint32_t K, uint32_t* scratch,
-                      hipStream_t s);
-void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s);
+void launch_duplicate(const DupArgs& a, hipStream_t s);
+void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s,
+                        const uint32_t* K_dev = nullptr);
+void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+
+// ---- multi-view per-Gaussian stages ----
+// Component-major scene parameters (channels x P).
+#define DMGS_MV_XYZ 0
+#define DMGS_MV_SCALE 3
+#define DMGS_MV_ROT 6
+#define DMGS_MV_OPACITY 10
+#define DMGS_MV_SH 11
+#define DMGS_MV_CHANNELS 59
+struct MVArgs {
+  int P, B, sh_degree;
+  float scale_modifier;
+  const ViewDesc* views;
+  const float* params;       // (59, P)
+  float4* splats;            // (B*P) records
+  int32_t* radii;            // (B*P)
+  uint32_t* tiles;           // (B*P)
+  uint32_t* depth_keys;      // (B*P)
+  uint32_t* order;           // (B*P)
+  float* gacc;               // (B*P, 16): cleared for visible entries by the forward
+  float* grads;              // (59, P)
+};
+void launch_mv_preprocess(const MVArgs& a, hipStream_t s);
+void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s);
 
 // ---- blend ----
 // Single view: W, H, grid_x, grid_y, bg and the image pointers describe the

@@ -147,8 +147,10 @@ void launch_inclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t
 template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, size_t n,
                                                                  int shift, int nbits,
-                                                                 uint32_t* __restrict__ hist, uint32_t nb) {
+                                                                 uint32_t* __restrict__ hist, uint32_t nb,
+                                                                 const uint32_t* __restrict__ n_dev) {
   __shared__ uint32_t cnt[256];
+  if (n_dev) n = min(n, (size_t)*n_dev);  // device-side length (blocks past it count nothing)
   const uint32_t mask = (1u << nbits) - 1u;
   cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -195,9 +197,11 @@ template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, size_t n, int shift, int nbits, const uint32_t* __restrict__ hist_excl,
-    const uint32_t* __restrict__ totals, uint32_t nb) {
+    const uint32_t* __restrict__ totals, uint32_t nb, const uint32_t* __restrict__ n_dev) {
   constexpr int kTile = kSortThreads * ITEMS;
   constexpr int kSlice = 64 * ITEMS;
+  if (n_dev) n = min(n, (size_t)*n_dev);
+  if ((size_t)blockIdx.x * kTile >= n) return;  // uniform per block
   __shared__ uint32_t wcnt[kSortWaves][256];
   __shared__ uint32_t dstart[256];  // block-local start of each digit
   __shared__ uint32_t gbase[256];   // global position of the block's first element of each digit
@@ -290,7 +294,7 @@ size_t radix_scratch_words(size_t n) {
 
 template <int ITEMS>
 static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
-                         uint32_t* scratch, hipStream_t s) {
+                         uint32_t* scratch, hipStream_t s, const uint32_t* n_dev) {
   constexpr size_t kTile = (size_t)kSortThreads * ITEMS;
   const uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
   const int passes = bits <= 16 ? 2 : 4;
@@ -308,10 +312,11 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
   for (int p = 0; p < passes; ++p) {
     const int w = widths[p];
     hipLaunchKernelGGL(radix_hist_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka, n, shift,
-                       w, hist, nb);
+                       w, hist, nb, n_dev);
     hipLaunchKernelGGL(radix_rowscan_kernel, dim3(1u << w), dim3(kScanThreads), 0, s, hist, nb, totals);
     hipLaunchKernelGGL(radix_scatter_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka,
-                       (const uint32_t*)va, kb, vb, n, shift, w, (const uint32_t*)hist, (const uint32_t*)totals, nb);
+                       (const uint32_t*)va, kb, vb, n, shift, w, (const uint32_t*)hist, (const uint32_t*)totals, nb,
+                       n_dev);
     shift += w;
     uint32_t* t = ka;
     ka = kb;
@@ -323,14 +328,14 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
 }
 
 void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
-                      uint32_t* scratch, hipStream_t s) {
+                      uint32_t* scratch, hipStream_t s, const uint32_t* n_dev) {
   if (n == 0) return;
   if (bits < 0) bits = 0;
   if (bits > 32) bits = 32;
   if (sort_items(n) == 16)
-    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s);
+    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s, n_dev);
   else
-    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s);
+    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s, n_dev);
 }
 
 // ---------------------------------------------------------------------------
@@ -356,38 +361,47 @@ void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, 
 // few huge splats cannot serialise a workgroup.
 constexpr uint32_t kDupChunk = 4096;
 
-__global__ __launch_bounds__(256) void chunk_first_kernel(int P, const uint32_t* __restrict__ offsets, uint32_t K,
-                                                          uint32_t* __restrict__ chunk_first) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= P) return;
-  const uint32_t beg = (s == 0) ? 0u : min(offsets[s - 1], K);
-  const uint32_t end = min(offsets[s], K);
-  for (uint32_t c = (beg + kDupChunk - 1) / kDupChunk; c * kDupChunk < end; ++c) chunk_first[c] = (uint32_t)s;
+__device__ __forceinline__ uint32_t device_count(uint32_t cap, const uint32_t* dev) {
+  return dev ? min(cap, *dev) : cap;
 }
 
-__global__ __launch_bounds__(256) void duplicate_kernel(int P, const uint32_t* __restrict__ order,
-                                                        const uint32_t* __restrict__ offsets,
-                                                        const float4* __restrict__ splats, int gx, int gy,
-                                                        uint32_t* __restrict__ tile_keys,
-                                                        uint32_t* __restrict__ values, uint32_t K,
-                                                        const uint32_t* __restrict__ chunk_first) {
-  __shared__ uint32_t s_end[256], s_g[256];
-  __shared__ int s_x0[256], s_y0[256], s_w[256];
+__global__ __launch_bounds__(256) void chunk_first_kernel(DupArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.P) return;
+  const uint32_t K = device_count(a.K, a.K_dev);
+  const uint32_t beg = (s == 0) ? 0u : min(a.offsets[s - 1], K);
+  const uint32_t end = min(a.offsets[s], K);
+  for (uint32_t c = (beg + kDupChunk - 1) / kDupChunk; c * kDupChunk < end; ++c) a.chunk_first[c] = (uint32_t)s;
+}
+
+__global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
+  __shared__ uint32_t s_end[256], s_g[256], s_tb[256];
+  __shared__ int s_x0[256], s_y0[256], s_w[256], s_gx[256];
+  const uint32_t K = device_count(a.K, a.K_dev);
   const uint32_t lo_slot = blockIdx.x * kDupChunk;
+  if (lo_slot >= K) return;  // uniform per block
   const uint32_t hi_slot = min(K, lo_slot + kDupChunk);
-  uint32_t first = chunk_first[blockIdx.x];
+  const uint32_t P = (uint32_t)a.P;
+  uint32_t first = a.chunk_first[blockIdx.x];
   while (true) {
     // stage Gaussians first .. first+255 (those past P get an empty range)
     const uint32_t s = first + threadIdx.x;
-    uint32_t end = 0xFFFFFFFFu, g = 0;
-    int x0 = 0, y0 = 0, w = 1;
-    if (s < (uint32_t)P) {
-      g = order[s];
-      const uint32_t beg = (s == 0) ? 0u : min(offsets[s - 1], K);
-      end = min(offsets[s], K);
-      if (end > beg && g < (uint32_t)P) {
-        const float4 s0 = splats[(size_t)DMGS_SPLAT_F4 * g];
-        const int r = __float_as_int(splats[(size_t)DMGS_SPLAT_F4 * g + 2].z);
+    uint32_t end = 0xFFFFFFFFu, g = 0, tb = 0;
+    int x0 = 0, y0 = 0, w = 1, gx = a.grid_x;
+    if (s < P) {
+      g = a.order[s];
+      const uint32_t beg = (s == 0) ? 0u : min(a.offsets[s - 1], K);
+      end = min(a.offsets[s], K);
+      if (end > beg && g < P) {
+        int gy = a.grid_y;
+        if (a.views) {
+          const ViewDesc& v = a.views[g / a.per_view];
+          gx = v.grid_x;
+          gy = v.grid_y;
+          tb = (uint32_t)v.tile_base;
+        }
+        const float4 s0 = a.splats[(size_t)DMGS_SPLAT_F4 * g];
+        const int r = __float_as_int(a.splats[(size_t)DMGS_SPLAT_F4 * g + 2].z);
         int2 rmin, rmax;
         tile_rect(make_float2(s0.x, s0.y), r, gx, gy, rmin, rmax);
         x0 = rmin.x;
@@ -397,11 +411,13 @@ __global__ __launch_bounds__(256) void duplicate_kernel(int P, const uint32_t* _
     }
     s_end[threadIdx.x] = end;
     s_g[threadIdx.x] = g;
+    s_tb[threadIdx.x] = tb;
     s_x0[threadIdx.x] = x0;
     s_y0[threadIdx.x] = y0;
     s_w[threadIdx.x] = w;
+    s_gx[threadIdx.x] = gx;
     __syncthreads();
-    const uint32_t win_begin = (first == 0) ? 0u : min(offsets[first - 1], K);
+    const uint32_t win_begin = (first == 0) ? 0u : min(a.offsets[first - 1], K);
     const uint32_t win_end = min(s_end[255], hi_slot);
     const uint32_t o_begin = max(lo_slot, win_begin);
     for (uint32_t o = o_begin + threadIdx.x; o < win_end; o += 256) {
@@ -415,10 +431,11 @@ __global__ __launch_bounds__(256) void duplicate_kernel(int P, const uint32_t* _
       const uint32_t local = o - beg;
       const uint32_t wj = (uint32_t)s_w[lo];
       const uint32_t row = local / wj;
-      tile_keys[o] = ((uint32_t)s_y0[lo] + row) * (uint32_t)gx + (uint32_t)s_x0[lo] + (local - row * wj);
-      values[o] = s_g[lo];
+      a.tile_keys[o] = s_tb[lo] + ((uint32_t)s_y0[lo] + row) * (uint32_t)s_gx[lo] + (uint32_t)s_x0[lo] +
+                       (local - row * wj);
+      a.values[o] = s_g[lo];
     }
-    const bool more = win_end < hi_slot && first + 256 < (uint32_t)P;
+    const bool more = win_end < hi_slot && first + 256 < P;
     __syncthreads();
     if (!more) break;
     first += 256;
@@ -427,14 +444,11 @@ __global__ __launch_bounds__(256) void duplicate_kernel(int P, const uint32_t* _
 
 size_t duplicate_scratch_words(size_t K) { return (K + kDupChunk - 1) / kDupChunk + 64; }
 
-void launch_duplicate(int P, const uint32_t* order, const uint32_t* offsets, const float4* splats, int grid_x,
-                      int grid_y, uint32_t* tile_keys, uint32_t* values, uint32_t K, uint32_t* scratch,
-                      hipStream_t s) {
-  if (P <= 0 || K == 0) return;
-  hipLaunchKernelGGL(chunk_first_kernel, dim3((P + 255) / 256), dim3(256), 0, s, P, offsets, K, scratch);
-  const uint32_t chunks = (K + kDupChunk - 1) / kDupChunk;
-  hipLaunchKernelGGL(duplicate_kernel, dim3(chunks), dim3(256), 0, s, P, order, offsets, splats, grid_x, grid_y,
-                     tile_keys, values, K, (const uint32_t*)scratch);
+void launch_duplicate(const DupArgs& a, hipStream_t s) {
+  if (a.P <= 0 || a.K == 0) return;
+  hipLaunchKernelGGL(chunk_first_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  const uint32_t chunks = (a.K + kDupChunk - 1) / kDupChunk;
+  hipLaunchKernelGGL(duplicate_kernel, dim3(chunks), dim3(256), 0, s, a);
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, size_t n, uint32_t v) {
@@ -442,8 +456,13 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, size_t n, uint32_t v) 
   if (i < n) p[i] = v;
 }
 
+void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
 __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, uint2* __restrict__ ranges,
-                                   uint32_t num_tiles) {
+                                   uint32_t num_tiles, const uint32_t* __restrict__ K_dev) {
+  if (K_dev) K = min(K, (size_t)*K_dev);
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= K) return;
   const uint32_t t = keys[i];
@@ -460,12 +479,11 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, 
   if (i == K - 1) ranges[t].y = (uint32_t)K;
 }
 
-void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s) {
-  const size_t words = (size_t)num_tiles * 2;
-  if (words) hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s,
-                                (uint32_t*)ranges, words, 0u);
+void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s,
+                        const uint32_t* K_dev) {
+  launch_fill_u32((uint32_t*)ranges, (size_t)num_tiles * 2, 0u, s);
   if (K) hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, tile_keys, K,
-                            ranges, (uint32_t)num_tiles);
+                            ranges, (uint32_t)num_tiles, K_dev);
 }
 
 }  // namespace dmgs

@@ -202,6 +202,81 @@ int dmgs_dssim_l1_forward(const float* img, const float* gt, int32_t C, int32_t 
 int dmgs_dssim_l1_backward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
                            const void* ws, const float* dloss, float* dimg, dmgs_stream_t stream);
 
+/* ---- batched views --------------------------------------------------------
+ * Replaces the per-view loop of the training / evaluation scripts
+ * (train_geo_stage3.py:104-124: one render() + loss per camera, one
+ * num_rendered host read-back per view) with one launch sequence over B views
+ * of one scene, with no host synchronisation, that a caller may capture in a
+ * HIP graph (kernels only: no memset or memcpy nodes).
+ *
+ * Scene parameters are component-major, params[c * P + g] for 59 channels:
+ *   0-2 xyz, 3-5 log-scale (get_scaling = exp), 6-9 raw quaternion (w,x,y,z;
+ *   get_rotation = normalise), 10 opacity logit (get_opacity = sigmoid),
+ *   11 + 3k + c SH coefficient k of colour c (k = 0 is features_dc, 1..15
+ *   features_rest), as scene/gaussian_model.py:26-41,95-118 store and
+ *   activate them.  The activations run inside the kernels and the backward
+ *   returns gradients of the raw channels (same layout), summed over views.
+ * Every per-view quantity follows the single-view path exactly: images are
+ * those of dmgs_forward for the same camera, and the gradients are the sum
+ * of the per-view gradients chained through the activations.
+ *
+ * Views may have different resolutions; each owns the tile range
+ * [tile_base, tile_base + grid_x*grid_y) and the pixel range
+ * [pixel_base, pixel_base + width*height) of the packed per-pixel buffers.
+ * Images are packed per view as (3, height, width) at offset 3*pixel_base. */
+typedef struct dmgs_view {
+  float viewmatrix[16]; /* world_view_transform, reference storage order */
+  float projmatrix[16]; /* full_proj_transform */
+  float campos[4];      /* xyz, then padding */
+  float bg[4];          /* rgb, then padding */
+  float tanfovx, tanfovy;
+  float focal_x, focal_y;   /* filled by dmgs_batched_prepare_views */
+  int32_t width, height;
+  int32_t grid_x, grid_y;   /* filled */
+  int32_t tile_base;        /* filled */
+  int32_t reserved;
+  int64_t pixel_base;       /* filled */
+} dmgs_view;
+
+/* Host-side: fill the derived fields of B host records and return the tile
+ * and pixel totals of the batch. */
+int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles, int64_t* total_pixels);
+/* Host-side: tile_view[t] = view owning global tile t (host array of total_tiles). */
+int dmgs_batched_tile_view(const dmgs_view* views, int32_t B, int32_t* tile_view, int32_t total_tiles);
+/* Workspace for B views, P Gaussians and room for K_cap (Gaussian, tile)
+ * instances over the whole batch. */
+size_t dmgs_batched_workspace_size(int32_t B, int64_t P, int64_t K_cap, int32_t total_tiles, int64_t total_pixels);
+/* Byte offsets inside the workspace: [0] radii (B*P int32, view-major),
+ * [1] instance count of the last forward (uint32), [2] tile ranges (uint2 per
+ * global tile), [3] final_T, [4] n_contrib (per packed pixel), [5] splat records. */
+int dmgs_batched_offsets(int32_t B, int64_t P, int64_t K_cap, int32_t total_tiles, int64_t total_pixels,
+                         size_t offsets[6]);
+/* Stage mask of the batched calls: the per-Gaussian stages (forward:
+ * preprocess, depth sort, scan, duplicate, tile sort, ranges; backward: the
+ * per-Gaussian backward) and the blend.  Callers normally pass
+ * DMGS_STAGES_ALL; a split lets a profiler bracket the blend on its own. */
+#define DMGS_STAGE_GEOMETRY 1
+#define DMGS_STAGE_BLEND 2
+#define DMGS_STAGES_ALL 3
+/* views / tile_view: device copies of the prepared records and of the tile
+ * map.  With K_cap == 0 only preprocess, depth sort and the scan run, so the
+ * instance count can be read to size K_cap.  When the count exceeds K_cap the
+ * lists are truncated (the images are then wrong): callers read the count
+ * back after the fact and re-plan.  variant selects the blend work-group
+ * shape (0..3, same results; an autotuner picks one). */
+int dmgs_batched_forward(const dmgs_view* views, int32_t B, int64_t P, int32_t sh_degree, float scale_modifier,
+                         const float* params, int64_t K_cap, int32_t total_tiles, int64_t total_pixels,
+                         const int32_t* tile_view, void* ws, float* images, int32_t variant, int32_t stages,
+                         dmgs_stream_t stream);
+/* Gradients of the 59 raw channels (59 x P, overwritten) from dL/dimages
+ * (packed like images), using the lists the forward left in ws.  The blend
+ * stage accumulates per (view, Gaussian); the geometry stage reads that and
+ * writes grads, so run the blend stage first when splitting. */
+int dmgs_batched_backward(const dmgs_view* views, int32_t B, int64_t P, int32_t sh_degree, float scale_modifier,
+                          const float* params, int64_t K_cap, int32_t total_tiles, int64_t total_pixels,
+                          const int32_t* tile_view, void* ws, const float* dL_dimages, float* grads, int32_t variant,
+                          int32_t stages, dmgs_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,122 @@
+"""Host-side logic of the batched pipeline (CPU only, no compute call):
+view preparation, the tile -> view map, workspace sizing, argument checks,
+and the component-major parameter packing."""
+import ctypes
+import math
+import os
+
+import pytest
+import torch
+
+from dmgs_amd import _native as N
+from dmgs_amd import batched as BT
+from dmgs_amd import cameras as C
+from dmgs_amd import scenes as S
+
+
+def _lib():
+    if not os.path.exists(N.library_path()):
+        pytest.skip("library not built (run __graft_entry__.build())")
+    return N.load()
+
+
+def _views(sizes):
+    cams = [C.camera_at(p, w, h, device="cpu") for p, (w, h) in zip(C.sphere_positions(len(sizes), 4.0, 0), sizes)]
+    return cams, (N.DmgsView * len(cams))(*[BT._view_record(c, torch.ones(3)) for c in cams])
+
+
+def test_prepare_views_and_tile_map():
+    lib = _lib()
+    sizes = [(800, 800), (1920, 1080), (33, 17), (16, 16)]
+    cams, views = _views(sizes)
+    tt, tp = N.c_int32(0), N.c_int64(0)
+    N.check(lib.dmgs_batched_prepare_views(views, len(sizes), ctypes.byref(tt), ctypes.byref(tp)), "prepare")
+    tiles = pixels = 0
+    for v, (w, h) in enumerate(sizes):
+        d = views[v]
+        assert (d.grid_x, d.grid_y) == ((w + 15) // 16, (h + 15) // 16)
+        assert d.tile_base == tiles and d.pixel_base == pixels
+        assert d.focal_x == pytest.approx(w / (2 * math.tan(cams[v].FoVx / 2)), rel=1e-6)
+        assert d.focal_y == pytest.approx(h / (2 * math.tan(cams[v].FoVy / 2)), rel=1e-6)
+        tiles += d.grid_x * d.grid_y
+        pixels += w * h
+    assert (tt.value, tp.value) == (tiles, pixels)
+    tv = torch.full((tiles,), -1, dtype=torch.int32)
+    N.check(lib.dmgs_batched_tile_view(views, len(sizes), ctypes.c_void_p(tv.data_ptr()), tiles), "tile_view")
+    expect = torch.cat([torch.full((views[v].grid_x * views[v].grid_y,), v, dtype=torch.int32)
+                        for v in range(len(sizes))])
+    assert torch.equal(tv, expect)
+    # a tile map shorter than the views' tiles is rejected
+    assert lib.dmgs_batched_tile_view(views, len(sizes), ctypes.c_void_p(tv.data_ptr()), tiles - 1) == -1
+
+
+def test_prepare_rejects_bad_views():
+    lib = _lib()
+    _, views = _views([(64, 64), (32, 32)])
+    views[1].width = 0
+    tt, tp = N.c_int32(0), N.c_int64(0)
+    assert lib.dmgs_batched_prepare_views(views, 2, ctypes.byref(tt), ctypes.byref(tp)) == -1
+    assert "view 1" in N.last_error()
+
+
+def test_workspace_and_offsets():
+    lib = _lib()
+    B, P, K, T, X = 3, 100000, 2_000_000, 7500, 1_920_000
+    n = lib.dmgs_batched_workspace_size(B, P, K, T, X)
+    # at least every compulsory buffer: 48 B splat + 7 x 4 B per entry, 4 x 4 B per instance,
+    # 8 B range per tile, 8 B per pixel, 64 B accumulator per entry
+    assert n >= B * P * (48 + 28 + 64) + K * 16 + T * 8 + X * 8
+    assert lib.dmgs_batched_workspace_size(0, P, K, T, X) == 0
+    assert lib.dmgs_batched_workspace_size(B, P, -1, T, X) == 0
+    offs = (N.c_size_t * 6)()
+    N.check(lib.dmgs_batched_offsets(B, P, K, T, X, offs), "offsets")
+    assert all(o % 4 == 0 and o < n for o in offs)
+    assert len(set(offs)) == 6
+    # the count location does not depend on the capacity (counting pass uses K = 0)
+    offs0 = (N.c_size_t * 6)()
+    N.check(lib.dmgs_batched_offsets(B, P, 0, T, X, offs0), "offsets0")
+    assert offs0[0] == offs[0] and offs0[1] == offs[1]
+
+
+def test_entry_points_validate_before_launch():
+    lib = _lib()
+    fake = ctypes.c_void_p(0x1000)
+    A = BT.STAGES_ALL
+    # no views / params
+    assert lib.dmgs_batched_forward(None, 2, 100, 3, 1.0, None, 10, 4, 100, fake, fake, fake, 0, A, None) == -1
+    # too many entries for 32-bit indices
+    assert lib.dmgs_batched_forward(fake, 4096, 1 << 20, 3, 1.0, fake, 10, 4, 100, fake, fake, fake, 0, A,
+                                    None) == -1
+    assert "invalid batch" in N.last_error()
+    assert lib.dmgs_batched_forward(fake, 2, 100, 4, 1.0, fake, 10, 4, 100, fake, fake, fake, 0, A, None) == -1
+    assert "sh_degree" in N.last_error()
+    for bad in (0, 4, -1):
+        assert lib.dmgs_batched_forward(fake, 2, 100, 3, 1.0, fake, 10, 4, 100, fake, fake, fake, 0, bad,
+                                        None) == -1
+        assert "stage mask" in N.last_error()
+    # the blend alone needs a capacity
+    assert lib.dmgs_batched_forward(fake, 2, 100, 3, 1.0, fake, 0, 4, 100, fake, fake, fake, 0, BT.STAGE_BLEND,
+                                    None) == -1
+    # backward needs a capacity and every buffer
+    assert lib.dmgs_batched_backward(fake, 2, 100, 3, 1.0, fake, 0, 4, 100, fake, fake, fake, fake, 0, A,
+                                     None) == -1
+    assert lib.dmgs_batched_backward(fake, 2, 100, 3, 1.0, fake, 10, 4, 100, fake, fake, None, fake, 0, A,
+                                     None) == -1
+    assert lib.dmgs_batched_backward(fake, 2, 100, 3, 1.0, fake, 10, 4, 100, fake, fake, fake, fake, 0, 8,
+                                     None) == -1
+
+
+def test_pack_unpack_round_trip():
+    m = S.free_scene(257, seed=3, sh_degree=3)
+    p = BT.pack_model(m)
+    assert p.shape == (BT.CHANNELS, 257) and p.is_contiguous()
+    assert torch.equal(p[BT.OPACITY], m._opacity[:, 0])
+    assert torch.equal(p[BT.SH + 3 * 5 + 2], m._features_rest[:, 4, 2])
+    back = BT.unpack_grads(p)
+    for k in ("_xyz", "_scaling", "_rotation", "_opacity", "_features_dc", "_features_rest"):
+        assert torch.equal(back[k], getattr(m, k)), k
+    # lower SH degrees are zero-padded to 16 coefficients
+    m1 = S.free_scene(10, seed=4, sh_degree=1)
+    p1 = BT.pack_model(m1)
+    assert torch.equal(p1[BT.SH + 12:], torch.zeros(48 - 12, 10))
+    assert torch.equal(BT.unpack_grads(p1, sh_coeffs=4)["_features_rest"], m1._features_rest)

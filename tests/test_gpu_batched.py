@@ -1,0 +1,253 @@
+"""GPU parity of the batched multi-view pipeline (MI355X).
+
+* Images of every view equal the drop-in single-view ``render`` of the same
+  camera (1e-4 abs; they differ only in where the activations are computed),
+  and the FP64 oracle's.
+* Raw-parameter gradients equal the sum over views of the single-view
+  gradients chained through the model activations by autograd, and the
+  oracle's chained the same way (1e-3 norm-relative per tensor).
+* The instance count equals the sum of the single-view counts; a capacity
+  below it is reported by ``overflowed`` and the truncated run stays in bounds.
+* Every blend variant gives bit-identical images.
+* A captured HIP graph of the whole step reproduces the eager step on every
+  replay, with eager work and synchronisation between replays.
+"""
+import numpy as np
+import pytest
+import torch
+
+from dmgs_amd import batched as BT
+from dmgs_amd import cameras as C
+from dmgs_amd import scenes as S
+from dmgs_amd.gaussian_renderer import PipelineParams, render
+from oracle import raster as OR
+from tests.helpers import grad_rel_err
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-4
+GRAD_TOL = 1e-3
+RAW = ("_xyz", "_scaling", "_rotation", "_opacity", "_features_dc", "_features_rest")
+
+
+def _cameras(sizes, seed=2, radius=4.0, device="cuda"):
+    pos = C.sphere_positions(len(sizes), radius, seed)
+    return [C.camera_at(p, w, h, fov_deg=50.0, device=device) for p, (w, h) in zip(pos, sizes)]
+
+
+def _bgs(n):
+    g = torch.Generator().manual_seed(5)
+    return [torch.rand(3, generator=g) for _ in range(n)]
+
+
+def _dL(renderer, seed=3):
+    g = torch.Generator().manual_seed(seed)
+    return (torch.randn(3 * renderer.total_pixels, generator=g) * 1e-3).to(renderer.device)
+
+
+def _plain(model_cpu, cams, bgs, dL_views):
+    """Single-view drop-in render per camera; autograd sums the gradients."""
+    m = model_cpu.to("cuda").requires_grad_(True)
+    pipe = PipelineParams()
+    imgs = []
+    counts = []
+    for cam, bg, dl in zip(cams, bgs, dL_views):
+        out = render(cam, m, pipe, bg.cuda())
+        (out["render"] * dl).sum().backward()
+        imgs.append(out["render"].detach())
+        counts.append(out.get("num_rendered"))
+    return imgs, {k: getattr(m, k).grad for k in RAW}, counts
+
+
+def _ambiguous(model_cpu, cam, bg):
+    """Pixels the FP64 oracle flags as blend threshold ties for this view
+    (alpha ~ 1/255, T ~ 1e-4, power ~ 0): FP32 paths that differ by an ulp
+    may resolve them differently, so they stay out of the comparison."""
+    m = model_cpu
+    st = OR.settings_for(_cpu(cam), bg.double(), 3)
+    _, _, state = OR.rasterize(m.get_xyz, m.get_opacity, st, shs=m.get_features, scales=m.get_scaling,
+                               rotations=m.get_rotation)
+    return state["ambiguous"]
+
+
+def _cpu(cam):
+    class Cam:
+        pass
+    c = Cam()
+    for k in ("image_width", "image_height", "FoVx", "FoVy"):
+        setattr(c, k, getattr(cam, k))
+    for k in ("world_view_transform", "full_proj_transform", "camera_center"):
+        setattr(c, k, getattr(cam, k).detach().cpu())
+    return c
+
+
+def _batched(model_cpu, cams, bgs, variant=0):
+    r = BT.BatchedRenderer(cams, [b.cuda() for b in bgs], model_cpu.num, sh_degree=model_cpu.max_sh_degree,
+                           variant=variant)
+    params = BT.pack_model(model_cpu.to("cuda"))
+    r.plan(params)
+    return r, params
+
+
+@pytest.mark.parametrize("n,sizes", [(20000, [(128, 96), (200, 200), (160, 90)]),
+                                     (5000, [(64, 48)]),
+                                     (30000, [(256, 256), (96, 160), (33, 17), (200, 120)])])
+def test_batched_matches_single_view(n, sizes):
+    print(f"case: batched vs single-view n={n} sizes={sizes}", flush=True)
+    model = S.free_scene(n, seed=0, layout="mixture", shell_radius=3.0)
+    cams = _cameras(sizes)
+    bgs = _bgs(len(cams))
+    r, params = _batched(model, cams, bgs)
+    dL = _dL(r)
+    amb = [_ambiguous(model, c, b) for c, b in zip(cams, bgs)]
+    for v in range(r.B):  # keep threshold ties out of the gradient comparison
+        r.image(dL, v)[:, amb[v].to(r.device)] = 0.0
+    images = r.forward(params)
+    assert not r.overflowed()
+    grads = r.backward(params, dL)
+    torch.cuda.synchronize()
+    imgs, pgrads, _ = _plain(model, cams, bgs, [r.image(dL, v) for v in range(r.B)])
+    for v in range(r.B):
+        ok = ~amb[v].to(r.device)
+        err = (r.image(images, v) - imgs[v]).abs()[:, ok].max().item()
+        print(f"view {v}: {int((~ok).sum())} tie pixels, max image error {err:.3g}", flush=True)
+        assert err < IMG_TOL, f"view {v}: image error {err}"
+    bg = BT.unpack_grads(grads)
+    for k in RAW:
+        e = grad_rel_err(bg[k], pgrads[k])
+        assert e < GRAD_TOL, f"{k}: rel err {e}"
+
+
+def test_batched_against_oracle():
+    print("case: batched vs FP64 oracle, 800 Gaussians, 2 views", flush=True)
+    model = S.free_scene(800, seed=4, layout="uniform")
+    sizes = [(64, 48), (40, 56)]
+    cams = _cameras(sizes, seed=7, radius=3.5, device="cpu")
+    bgs = _bgs(2)
+    r, params = _batched(model, cams, bgs)
+    dL = _dL(r, seed=9)
+    # oracle on FP64 activated parameters; activations chained by autograd
+    raw = {k: getattr(model, k).detach().double().requires_grad_(True) for k in RAW}
+    act = dict(means3D=raw["_xyz"], opac=torch.sigmoid(raw["_opacity"]),
+               shs=torch.cat((raw["_features_dc"], raw["_features_rest"]), 1), scales=torch.exp(raw["_scaling"]),
+               rots=torch.nn.functional.normalize(raw["_rotation"]))
+    runs = []
+    for v, cam in enumerate(cams):
+        st = OR.settings_for(cam, bgs[v].double(), 3)
+        color, _, state = OR.rasterize(act["means3D"].detach(), act["opac"].detach(), st,
+                                       shs=act["shs"].detach(), scales=act["scales"].detach(),
+                                       rotations=act["rots"].detach())
+        r.image(dL, v)[:, state["ambiguous"].to(r.device)] = 0.0  # threshold ties stay out
+        runs.append((st, color, state))
+    images = r.forward(params)
+    grads = BT.unpack_grads(r.backward(params, dL))
+    torch.cuda.synchronize()
+    total = {k: torch.zeros_like(v) for k, v in act.items()}
+    for v, (st, color, state) in enumerate(runs):
+        ok = ~state["ambiguous"]
+        err = (r.image(images, v).cpu().double() - color).abs()[:, ok].max().item()
+        assert err < IMG_TOL, f"view {v}: image error vs oracle {err}"
+        g = OR.rasterize_backward(state, st, act["means3D"].detach(), r.image(dL, v).cpu().double(),
+                                  shs=act["shs"].detach(), scales=act["scales"].detach(),
+                                  rotations=act["rots"].detach())
+        dmeans3D, _, dsh, _, dop, dscales, drot = g[:7]
+        total["means3D"] += dmeans3D
+        total["shs"] += dsh
+        total["opac"] += dop.reshape(total["opac"].shape)
+        total["scales"] += dscales
+        total["rots"] += drot
+    torch.autograd.backward([act[k] for k in act], [total[k] for k in act])
+    for k in RAW:
+        e = grad_rel_err(grads[k], raw[k].grad)
+        assert e < GRAD_TOL, f"{k}: rel err vs oracle {e}"
+
+
+def test_instance_count_and_overflow():
+    print("case: instance count and overflow detection", flush=True)
+    model = S.free_scene(20000, seed=1, layout="mixture")
+    sizes = [(128, 128), (96, 64)]
+    cams = _cameras(sizes, seed=11)
+    bgs = _bgs(2)
+    r, params = _batched(model, cams, bgs)
+    K = r.count_instances(params)
+    # sum of the single-view counts (binning of the drop-in path)
+    from dmgs_amd import rasterizer as RZ
+    from tests.helpers import settings_from_cam
+    m = model.to("cuda")
+    Ks = []
+    for cam, bg in zip(cams, bgs):
+        rs = settings_from_cam(cam, bg, 3)
+        _, _, st = RZ.forward_raw(m.get_xyz, m.get_opacity, m.get_features, None, m.get_scaling, m.get_rotation,
+                                  None, rs)
+        Ks.append(st["K"])
+    assert K == sum(Ks)
+    images_full = r.forward(params).clone()
+    assert not r.overflowed()
+    # a capacity below the count: reported, truncated, stays in bounds
+    r.plan(params, K_cap=K // 2)
+    images_cut = r.forward(params)
+    torch.cuda.synchronize()
+    assert r.overflowed()
+    assert torch.isfinite(images_cut).all()
+    r.plan(params)
+    images_again = r.forward(params)
+    assert not r.overflowed()
+    assert torch.equal(images_again, images_full)
+
+
+def test_render_variants_identical():
+    print("case: blend variants", flush=True)
+    model = S.free_scene(20000, seed=2, layout="mixture")
+    cams = _cameras([(200, 150), (77, 53)], seed=13)
+    bgs = _bgs(2)
+    r, params = _batched(model, cams, bgs)
+    dL = _dL(r)
+    ref_img = r.forward(params).clone()
+    ref_grad = r.backward(params, dL).clone()
+    for v in range(1, len(BT.RENDER_VARIANTS)):
+        r.variant = v
+        img = r.forward(params)
+        grad = r.backward(params, dL)
+        torch.cuda.synchronize()
+        assert torch.equal(img, ref_img), f"variant {v} image differs"
+        assert grad_rel_err(grad, ref_grad) < 1e-5, f"variant {v} gradient differs"
+    times = r.autotune(params, dL, reps=1)
+    assert set(times) == set(range(len(BT.RENDER_VARIANTS)))
+
+
+def test_graph_replay_matches_eager():
+    print("case: HIP graph replay of the batched step", flush=True)
+    model = S.free_scene(20000, seed=3, layout="mixture")
+    sizes = [(160, 120), (128, 128), (90, 60)]
+    cams = _cameras(sizes, seed=17)
+    bgs = _bgs(3)
+    r, params = _batched(model, cams, bgs)
+    targets = [S.target_image(h, w, seed=20 + i) for i, (w, h) in enumerate(sizes)]
+    step = BT.BatchedStep(r, BT.BatchedLoss(r, targets), params)
+    eager_loss = step.run().clone()
+    eager_img = step.images.clone()
+    eager_grad = step.grads.clone()
+    # reference: the single-view path with the fused loss, per view
+    from dmgs_amd.loss import fused_dssim_l1
+    m = model.to("cuda").requires_grad_(True)
+    for v, (cam, bg) in enumerate(zip(cams, bgs)):
+        out = render(cam, m, PipelineParams(), bg.cuda())
+        loss = fused_dssim_l1(out["render"], targets[v].cuda(), 0.2)
+        assert abs(loss.item() - eager_loss[v, 0].item()) < 1e-5
+        loss.backward()
+    g = BT.unpack_grads(eager_grad)
+    for k in RAW:
+        assert grad_rel_err(g[k], getattr(m, k).grad) < GRAD_TOL, k
+    step.capture()
+    for rep in range(4):
+        step.images.zero_()
+        step.grads.zero_()
+        _ = (torch.randn(1000, 1000, device="cuda") @ torch.randn(1000, 1000, device="cuda")).sum().item()
+        step.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(step.loss.losses, eager_loss), f"replay {rep}: losses differ"
+        assert torch.equal(step.images, eager_img), f"replay {rep}: images differ"
+        assert grad_rel_err(step.grads, eager_grad) < 1e-5, f"replay {rep}: gradients differ"
+        step.run()  # eager work between replays
+        torch.cuda.synchronize()
+    assert not r.overflowed()
