@@ -294,7 +294,7 @@ def main():
     timer = StageTimer()
     for _ in range(max(3, args.steps // 2)):
         with timer("geometry+binning"):
-            r.forward(params, step.images, stages=BT.STAGE_GEOMETRY)
+            r.forward(params, step.images, stages=BT.STAGE_GEOMETRY | BT.STAGE_BINNING)
         with timer("render_forward"):
             r.forward(params, step.images, stages=BT.STAGE_BLEND)
         with timer("loss"):

@@ -35,6 +35,16 @@ class DmgsView(ctypes.Structure):
     ]
 
 
+class DmgsBatch(ctypes.Structure):
+    """struct dmgs_batch (include/dmgs.h): a prepared batch and its workspaces."""
+    _fields_ = [
+        ("views", c_void_p), ("tile_view", c_void_p), ("num_views", c_int32), ("sh_degree", c_int32),
+        ("num_gaussians", c_int64), ("scale_modifier", c_float), ("variant", c_int32), ("total_tiles", c_int32),
+        ("reserved", c_int32), ("total_pixels", c_int64), ("capacity", c_int64),
+        ("state", c_void_p), ("lists", c_void_p), ("bwd_ws", c_void_p),
+    ]
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "dmgs_last_error": (ctypes.c_char_p, []),
@@ -74,12 +84,12 @@ _SIGNATURES = {
     "dmgs_batched_prepare_views": (c_int32, [ctypes.POINTER(DmgsView), c_int32, ctypes.POINTER(c_int32),
                                              ctypes.POINTER(c_int64)]),
     "dmgs_batched_tile_view": (c_int32, [ctypes.POINTER(DmgsView), c_int32, c_void_p, c_int32]),
-    "dmgs_batched_workspace_size": (c_size_t, [c_int32, c_int64, c_int64, c_int32, c_int64]),
-    "dmgs_batched_offsets": (c_int32, [c_int32, c_int64, c_int64, c_int32, c_int64, ctypes.POINTER(c_size_t)]),
-    "dmgs_batched_forward": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_float, c_void_p, c_int64, c_int32,
-                                       c_int64, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p]),
-    "dmgs_batched_backward": (c_int32, [c_void_p, c_int32, c_int64, c_int32, c_float, c_void_p, c_int64, c_int32,
-                                        c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
+    "dmgs_batched_state_size": (c_size_t, [c_int32, c_int64, c_int32, c_int64]),
+    "dmgs_batched_lists_size": (c_size_t, [c_int64]),
+    "dmgs_batched_backward_size": (c_size_t, [c_int32, c_int64]),
+    "dmgs_batched_offsets": (c_int32, [c_int32, c_int64, c_int32, c_int64, ctypes.POINTER(c_size_t)]),
+    "dmgs_batched_forward": (c_int32, [ctypes.POINTER(DmgsBatch), c_void_p, c_void_p, c_int32, c_void_p]),
+    "dmgs_batched_backward": (c_int32, [ctypes.POINTER(DmgsBatch), c_void_p, c_void_p, c_void_p, c_int32,
                                         c_void_p]),
 }
 

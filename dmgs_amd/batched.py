@@ -30,7 +30,7 @@ from . import _native as N
 
 CHANNELS = 59
 XYZ, SCALE, ROT, OPACITY, SH = 0, 3, 6, 10, 11
-STAGE_GEOMETRY, STAGE_BLEND, STAGES_ALL = 1, 2, 3
+STAGE_GEOMETRY, STAGE_BINNING, STAGE_BLEND, STAGES_ALL = 1, 2, 4, 7
 RENDER_VARIANTS = ("16x16 px, 256-deep LDS batch", "16x16 px, 128-deep", "8x8 px, 64-deep", "8x8 px, 128-deep")
 
 
@@ -125,38 +125,70 @@ class BatchedRenderer:
         self.sizes = [(v.height, v.width) for v in self.views]
         self.pixel_base = [v.pixel_base for v in self.views]
         self.K_cap = 0
-        self.ws = None
+        self.state = self.lists = self.bwd_ws = None
         self._offsets = None
 
     # ---- planning ---------------------------------------------------------
-    def _alloc(self, K_cap):
-        n = self.lib.dmgs_batched_workspace_size(self.B, self.P, K_cap, self.total_tiles, self.total_pixels)
+    def _alloc_state(self):
+        n = self.lib.dmgs_batched_state_size(self.B, self.P, self.total_tiles, self.total_pixels)
         if n == 0:
-            raise RuntimeError("dmgs_batched_workspace_size rejected the batch")
+            raise RuntimeError("dmgs_batched_state_size rejected the batch")
         offs = (N.c_size_t * 6)()
-        N.check(self.lib.dmgs_batched_offsets(self.B, self.P, K_cap, self.total_tiles, self.total_pixels, offs),
+        N.check(self.lib.dmgs_batched_offsets(self.B, self.P, self.total_tiles, self.total_pixels, offs),
                 "dmgs_batched_offsets")
-        return torch.empty(n, dtype=torch.uint8, device=self.device), list(offs)
+        self.state = torch.empty(n, dtype=torch.uint8, device=self.device)
+        self._offsets = list(offs)
+
+    def _desc(self, capacity=None):
+        d = N.DmgsBatch()
+        d.views, d.tile_view = self.views_dev.data_ptr(), self.tile_view.data_ptr()
+        d.num_views, d.sh_degree, d.num_gaussians = self.B, self.sh_degree, self.P
+        d.scale_modifier, d.variant = self.scale_modifier, self.variant
+        d.total_tiles, d.total_pixels = self.total_tiles, self.total_pixels
+        d.capacity = self.K_cap if capacity is None else capacity
+        d.state = self.state.data_ptr()
+        d.lists = None if self.lists is None else self.lists.data_ptr()
+        d.bwd_ws = None if self.bwd_ws is None else self.bwd_ws.data_ptr()
+        return d
 
     def count_instances(self, params):
         """Run preprocess + depth sort + scan only and return the number of
         (Gaussian, tile) instances of the batch (one host read)."""
-        ws, offs = (self.ws, self._offsets) if self.ws is not None else self._alloc(0)
-        self._call_forward(params, 0, ws, None)
-        return int(ws[offs[1]:offs[1] + 4].view(torch.int32).item()) & 0xFFFFFFFF
+        if self.state is None:
+            self._alloc_state()
+        self._check_params(params)
+        N.check(self.lib.dmgs_batched_forward(ctypes.byref(self._desc(0)), N.ptr(params), None, STAGE_GEOMETRY,
+                                              N.stream_handle(self.device)), "dmgs_batched_forward")
+        return int(self.instance_count().item()) & 0xFFFFFFFF
 
-    def plan(self, params, K_cap=None):
+    def count_instances_async(self, params, host_out):
+        """Launch preprocess + depth sort + scan and copy the instance count
+        into ``host_out`` (pinned int32 tensor of one element) without
+        waiting; the count is valid once the current stream has reached it."""
+        if self.state is None:
+            self._alloc_state()
+        self._check_params(params)
+        N.check(self.lib.dmgs_batched_forward(ctypes.byref(self._desc(0)), N.ptr(params), None, STAGE_GEOMETRY,
+                                              N.stream_handle(self.device)), "dmgs_batched_forward")
+        host_out.copy_(self.instance_count(), non_blocking=True)
+
+    def plan(self, params, K_cap=None, backward=True):
         """Size the instance lists for ``params`` (with headroom) and allocate
-        the workspace.  Returns the capacity."""
+        the workspaces.  Returns the capacity."""
+        if self.state is None:
+            self._alloc_state()
         if K_cap is None:
             K = self.count_instances(params)
             K_cap = max(1024, int(math.ceil(K * self.headroom)))
         self.K_cap = int(K_cap)
-        self.ws, self._offsets = self._alloc(self.K_cap)
+        self.lists = torch.empty(self.lib.dmgs_batched_lists_size(self.K_cap), dtype=torch.uint8, device=self.device)
+        if backward and self.bwd_ws is None:
+            self.bwd_ws = torch.empty(self.lib.dmgs_batched_backward_size(self.B, self.P), dtype=torch.uint8,
+                                      device=self.device)
         return self.K_cap
 
     def _require_plan(self):
-        if self.ws is None:
+        if self.lists is None:
             raise RuntimeError("BatchedRenderer.plan() must run before forward/backward")
 
     def _check_params(self, params):
@@ -166,39 +198,35 @@ class BatchedRenderer:
             raise ValueError("params must live on the renderer's device")
 
     # ---- launches ---------------------------------------------------------
-    def _call_forward(self, params, K_cap, ws, images, stages=STAGES_ALL):
-        self._check_params(params)
-        N.check(self.lib.dmgs_batched_forward(
-            N.ptr(self.views_dev), self.B, self.P, self.sh_degree, self.scale_modifier, N.ptr(params), K_cap,
-            self.total_tiles, self.total_pixels, N.ptr(self.tile_view), N.ptr(ws), N.ptr(images), self.variant,
-            stages, N.stream_handle(self.device)), "dmgs_batched_forward")
-
     def new_images(self):
         return torch.empty(3 * self.total_pixels, dtype=torch.float32, device=self.device)
 
     def forward(self, params, images=None, stages=STAGES_ALL):
         """Render every view into one packed buffer (view v at 3*pixel_base[v],
-        laid out (3, H, W)); returns it.  ``stages`` may split the call into
-        the per-Gaussian part and the blend (run in that order)."""
+        laid out (3, H, W)); returns it.  ``stages`` may split the call
+        (geometry, binning, blend; run in that order)."""
         self._require_plan()
+        self._check_params(params)
         if images is None:
             images = self.new_images()
-        self._call_forward(params, self.K_cap, self.ws, images, stages)
+        N.check(self.lib.dmgs_batched_forward(ctypes.byref(self._desc()), N.ptr(params), N.ptr(images), stages,
+                                              N.stream_handle(self.device)), "dmgs_batched_forward")
         return images
 
-    def backward(self, params, dL_dimages, grads=None, stages=STAGES_ALL):
+    def backward(self, params, dL_dimages, grads=None, stages=STAGE_BLEND | STAGE_GEOMETRY):
         """Raw-parameter gradients (59, P) of sum over views of <dL/dimage, image>
         (``stages``: blend backward, then the per-Gaussian backward)."""
         self._require_plan()
         self._check_params(params)
+        if self.bwd_ws is None:
+            raise RuntimeError("planned without backward")
         if dL_dimages.numel() != 3 * self.total_pixels or not dL_dimages.is_contiguous():
             raise ValueError("dL_dimages must be a contiguous buffer packed like the images")
         if grads is None:
             grads = torch.empty(CHANNELS, self.P, dtype=torch.float32, device=self.device)
-        N.check(self.lib.dmgs_batched_backward(
-            N.ptr(self.views_dev), self.B, self.P, self.sh_degree, self.scale_modifier, N.ptr(params), self.K_cap,
-            self.total_tiles, self.total_pixels, N.ptr(self.tile_view), N.ptr(self.ws), N.ptr(dL_dimages),
-            N.ptr(grads), self.variant, stages, N.stream_handle(self.device)), "dmgs_batched_backward")
+        N.check(self.lib.dmgs_batched_backward(ctypes.byref(self._desc()), N.ptr(params), N.ptr(dL_dimages),
+                                               N.ptr(grads), stages, N.stream_handle(self.device)),
+                "dmgs_batched_backward")
         return grads
 
     # ---- results ----------------------------------------------------------
@@ -210,9 +238,8 @@ class BatchedRenderer:
 
     def instance_count(self):
         """Device tensor (1,) holding the instance count of the last forward."""
-        self._require_plan()
         o = self._offsets[1]
-        return self.ws[o:o + 4].view(torch.int32)
+        return self.state[o:o + 4].view(torch.int32)
 
     def overflowed(self):
         """True when the last forward needed more instances than planned
@@ -221,9 +248,8 @@ class BatchedRenderer:
 
     def radii(self):
         """(B, P) int32 radii of the last forward."""
-        self._require_plan()
         o = self._offsets[0]
-        return self.ws[o:o + 4 * self.B * self.P].view(torch.int32).view(self.B, self.P)
+        return self.state[o:o + 4 * self.B * self.P].view(torch.int32).view(self.B, self.P)
 
     # ---- autotuning -------------------------------------------------------
     def autotune(self, params, dL_dimages=None, reps=3):

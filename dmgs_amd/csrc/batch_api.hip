@@ -31,6 +31,7 @@ static_assert(offsetof(dmgs_view, pixel_base) == offsetof(dmgs::ViewDesc, pixel_
 static_assert(offsetof(dmgs_view, tanfovx) == offsetof(dmgs::ViewDesc, tanfovx), "dmgs_view layout");
 static_assert(offsetof(dmgs_view, width) == offsetof(dmgs::ViewDesc, W), "dmgs_view layout");
 static_assert(offsetof(dmgs_view, bg) == offsetof(dmgs::ViewDesc, bg), "dmgs_view layout");
+static_assert(sizeof(dmgs_batch) == 88 && offsetof(dmgs_batch, capacity) == 56, "dmgs_batch layout");
 
 namespace {
 
@@ -63,11 +64,12 @@ int bits_for(uint32_t n) {
   return b;
 }
 
-struct BatchLayout {
-  size_t splats, keys_a, keys_b, order_a, order_b, tiles, radii, offsets, gscratch;
-  size_t tkeys_a, tkeys_b, vals_a, vals_b, bscratch;
-  size_t ranges, final_T, n_contrib, gacc, total;
-  BatchLayout(size_t N, size_t K, size_t T, size_t X) {
+// K-independent state of a batch: per-entry splats / keys / order / radii /
+// scan, the depth-sort scratch, per-tile ranges and per-pixel blend state.
+struct StateLayout {
+  size_t splats, keys_a, keys_b, order_a, order_b, tiles, radii, offsets, gscratch, ranges, final_T, n_contrib,
+      total;
+  StateLayout(size_t N, size_t T, size_t X) {
     size_t o = 0;
     splats = o; o += align_up(N * DMGS_SPLAT_F4 * 16);
     keys_a = o; o += align_up(N * 4);
@@ -78,60 +80,80 @@ struct BatchLayout {
     radii = o; o += align_up(N * 4);
     offsets = o; o += align_up(N * 4);
     gscratch = o; o += align_up(std::max(dmgs::radix_scratch_words(N), dmgs::scan_scratch_words(N)) * 4);
-    tkeys_a = o; o += align_up(K * 4);
-    tkeys_b = o; o += align_up(K * 4);
-    vals_a = o; o += align_up(K * 4);
-    vals_b = o; o += align_up(K * 4);
-    bscratch = o; o += align_up(std::max(dmgs::radix_scratch_words(K), dmgs::duplicate_scratch_words(K)) * 4);
     ranges = o; o += align_up(T * 8);
     final_T = o; o += align_up(X * 4);
     n_contrib = o; o += align_up(X * 4);
-    gacc = o; o += align_up(N * DMGS_GACC_STRIDE * 4);
     total = o;
   }
 };
 
-int check_batch(int32_t B, int64_t P, int64_t K_cap, int32_t total_tiles, int64_t total_pixels) {
-  if (B <= 0 || P <= 0 || (int64_t)B * P > (int64_t)0x7fffffff)
-    return bfail(DMGS_EINVAL, "invalid batch: %d views x %lld Gaussians", B, (long long)P);
-  if (K_cap < 0 || K_cap > (int64_t)0xffffffff) return bfail(DMGS_EINVAL, "invalid instance capacity");
-  if (total_tiles <= 0 || total_pixels <= 0) return bfail(DMGS_EINVAL, "invalid view totals");
+// Per-instance lists (tile keys and entry ids, ping-pong) and their scratch.
+struct ListLayout {
+  size_t tkeys_a, tkeys_b, vals_a, vals_b, scratch, total;
+  explicit ListLayout(size_t K) {
+    size_t o = 0;
+    tkeys_a = o; o += align_up(K * 4);
+    tkeys_b = o; o += align_up(K * 4);
+    vals_a = o; o += align_up(K * 4);
+    vals_b = o; o += align_up(K * 4);
+    scratch = o; o += align_up(std::max(dmgs::radix_scratch_words(K), dmgs::duplicate_scratch_words(K)) * 4);
+    total = o;
+  }
+};
+
+int check_batch(const dmgs_batch* b) {
+  if (!b) return bfail(DMGS_EINVAL, "batch is null");
+  if (b->num_views <= 0 || b->num_gaussians <= 0 || (int64_t)b->num_views * b->num_gaussians > (int64_t)0x7fffffff)
+    return bfail(DMGS_EINVAL, "invalid batch: %d views x %lld Gaussians", b->num_views,
+                 (long long)b->num_gaussians);
+  if (b->capacity < 0 || b->capacity > (int64_t)0xffffffff) return bfail(DMGS_EINVAL, "invalid instance capacity");
+  if (b->total_tiles <= 0 || b->total_pixels <= 0) return bfail(DMGS_EINVAL, "invalid view totals");
+  if (b->sh_degree < 0 || b->sh_degree > 3) return bfail(DMGS_EINVAL, "sh_degree must be in [0, 3]");
+  if (!b->views || !b->state) return bfail(DMGS_EINVAL, "views and state are required");
   return DMGS_OK;
 }
 
-dmgs::MVArgs mv_args(const dmgs_view* views, int32_t B, int64_t P, int32_t sh_degree, float mod,
-                     const float* params, void* ws, const BatchLayout& L) {
+struct Ctx {
+  size_t N;
+  StateLayout S;
+  ListLayout L;
+  explicit Ctx(const dmgs_batch* b)
+      : N((size_t)b->num_views * b->num_gaussians), S(N, (size_t)b->total_tiles, (size_t)b->total_pixels),
+        L((size_t)b->capacity) {}
+  uint32_t* count(const dmgs_batch* b) const { return at<uint32_t>(b->state, S.offsets) + (N - 1); }
+};
+
+dmgs::MVArgs mv_args(const dmgs_batch* b, const Ctx& c, const float* params) {
   dmgs::MVArgs a{};
-  a.P = (int)P;
-  a.B = B;
-  a.sh_degree = sh_degree;
-  a.scale_modifier = mod;
-  a.views = reinterpret_cast<const dmgs::ViewDesc*>(views);
+  a.P = (int)b->num_gaussians;
+  a.B = b->num_views;
+  a.sh_degree = b->sh_degree;
+  a.scale_modifier = b->scale_modifier;
+  a.views = reinterpret_cast<const dmgs::ViewDesc*>(b->views);
   a.params = params;
-  a.splats = at<float4>(ws, L.splats);
-  a.radii = at<int32_t>(ws, L.radii);
-  a.tiles = at<uint32_t>(ws, L.tiles);
-  a.depth_keys = at<uint32_t>(ws, L.keys_a);
-  a.order = at<uint32_t>(ws, L.order_a);
-  a.gacc = at<float>(ws, L.gacc);
+  a.splats = at<float4>(b->state, c.S.splats);
+  a.radii = at<int32_t>(b->state, c.S.radii);
+  a.tiles = at<uint32_t>(b->state, c.S.tiles);
+  a.depth_keys = at<uint32_t>(b->state, c.S.keys_a);
+  a.order = at<uint32_t>(b->state, c.S.order_a);
+  a.gacc = static_cast<float*>(b->bwd_ws);
   return a;
 }
 
-dmgs::RenderArgs render_args(const dmgs_view* views, size_t N, int64_t K_cap, int32_t total_tiles,
-                             const int32_t* tile_view, void* ws, const BatchLayout& L) {
+dmgs::RenderArgs render_args(const dmgs_batch* b, const Ctx& c) {
   dmgs::RenderArgs r{};
-  r.num_tiles = total_tiles;
-  r.P = (uint32_t)N;
-  r.K = (uint32_t)K_cap;
-  r.K_dev = at<uint32_t>(ws, L.offsets) + (N - 1);
-  r.views = reinterpret_cast<const dmgs::ViewDesc*>(views);
-  r.tile_view = tile_view;
-  r.ranges = at<uint2>(ws, L.ranges);
-  r.point_list = at<uint32_t>(ws, L.vals_a);
-  r.splats = at<float4>(ws, L.splats);
-  r.final_T = at<float>(ws, L.final_T);
-  r.n_contrib = at<uint32_t>(ws, L.n_contrib);
-  r.gacc = at<float>(ws, L.gacc);
+  r.num_tiles = b->total_tiles;
+  r.P = (uint32_t)c.N;
+  r.K = (uint32_t)b->capacity;
+  r.K_dev = c.count(b);
+  r.views = reinterpret_cast<const dmgs::ViewDesc*>(b->views);
+  r.tile_view = b->tile_view;
+  r.ranges = at<uint2>(b->state, c.S.ranges);
+  r.point_list = at<uint32_t>(b->lists, c.L.vals_a);
+  r.splats = at<float4>(b->state, c.S.splats);
+  r.final_T = at<float>(b->state, c.S.final_T);
+  r.n_contrib = at<uint32_t>(b->state, c.S.n_contrib);
+  r.gacc = static_cast<float*>(b->bwd_ws);
   return r;
 }
 
@@ -172,107 +194,115 @@ int dmgs_batched_tile_view(const dmgs_view* views, int32_t B, int32_t* tile_view
   return DMGS_OK;
 }
 
-size_t dmgs_batched_workspace_size(int32_t B, int64_t P, int64_t K_cap, int32_t total_tiles, int64_t total_pixels) {
-  if (B <= 0 || P <= 0 || K_cap < 0 || total_tiles <= 0 || total_pixels <= 0) return 0;
-  return BatchLayout((size_t)B * P, (size_t)K_cap, (size_t)total_tiles, (size_t)total_pixels).total;
+size_t dmgs_batched_state_size(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels) {
+  if (B <= 0 || P <= 0 || total_tiles <= 0 || total_pixels <= 0) return 0;
+  return StateLayout((size_t)B * P, (size_t)total_tiles, (size_t)total_pixels).total;
 }
 
-int dmgs_batched_offsets(int32_t B, int64_t P, int64_t K_cap, int32_t total_tiles, int64_t total_pixels,
-                       size_t offsets[6]) {
-  int rc = check_batch(B, P, K_cap, total_tiles, total_pixels);
-  if (rc) return rc;
+size_t dmgs_batched_lists_size(int64_t capacity) {
+  if (capacity <= 0) return 0;
+  return ListLayout((size_t)capacity).total;
+}
+
+size_t dmgs_batched_backward_size(int32_t B, int64_t P) {
+  if (B <= 0 || P <= 0) return 0;
+  return (size_t)B * P * DMGS_GACC_STRIDE * 4;
+}
+
+int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, size_t offsets[6]) {
+  if (B <= 0 || P <= 0 || total_tiles <= 0 || total_pixels <= 0 || (int64_t)B * P > (int64_t)0x7fffffff)
+    return bfail(DMGS_EINVAL, "invalid batch sizes");
   if (!offsets) return bfail(DMGS_EINVAL, "offsets is null");
   const size_t N = (size_t)B * P;
-  BatchLayout L(N, (size_t)K_cap, (size_t)total_tiles, (size_t)total_pixels);
-  offsets[0] = L.radii;                      // (B*P) int32 radii, view-major
-  offsets[1] = L.offsets + 4 * (N - 1);      // uint32 instance count of the batch
-  offsets[2] = L.ranges;
-  offsets[3] = L.final_T;
-  offsets[4] = L.n_contrib;
-  offsets[5] = L.splats;
+  StateLayout S(N, (size_t)total_tiles, (size_t)total_pixels);
+  offsets[0] = S.radii;                  // (B*P) int32 radii, view-major
+  offsets[1] = S.offsets + 4 * (N - 1);  // uint32 instance count of the batch
+  offsets[2] = S.ranges;
+  offsets[3] = S.final_T;
+  offsets[4] = S.n_contrib;
+  offsets[5] = S.splats;
   return DMGS_OK;
 }
 
-int dmgs_batched_forward(const dmgs_view* views, int32_t B, int64_t P, int32_t sh_degree, float scale_modifier,
-                       const float* params, int64_t K_cap, int32_t total_tiles, int64_t total_pixels,
-                       const int32_t* tile_view, void* ws, float* images, int32_t variant, int32_t stages,
-                       dmgs_stream_t stream) {
-  int rc = check_batch(B, P, K_cap, total_tiles, total_pixels);
+int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images, int32_t stages,
+                         dmgs_stream_t stream) {
+  int rc = check_batch(b);
   if (rc) return rc;
-  if (!views || !params || !ws || (K_cap > 0 && (!tile_view || !images)))
-    return bfail(DMGS_EINVAL, "views, params, ws, tile_view and images are required");
-  if (sh_degree < 0 || sh_degree > 3) return bfail(DMGS_EINVAL, "sh_degree must be in [0, 3]");
+  if (!params) return bfail(DMGS_EINVAL, "params is required");
   if (stages <= 0 || (stages & ~DMGS_STAGES_ALL)) return bfail(DMGS_EINVAL, "invalid stage mask %d", stages);
+  if ((stages & (DMGS_STAGE_BINNING | DMGS_STAGE_BLEND)) && (b->capacity == 0 || !b->lists || !b->tile_view))
+    return bfail(DMGS_EINVAL, "binning and blend need lists, a capacity and the tile map");
+  if ((stages & DMGS_STAGE_BLEND) && !images) return bfail(DMGS_EINVAL, "images is required for the blend");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const size_t N = (size_t)B * P;
-  BatchLayout L(N, (size_t)K_cap, (size_t)total_tiles, (size_t)total_pixels);
-  const dmgs::MVArgs a = mv_args(views, B, P, sh_degree, scale_modifier, params, ws, L);
-  if (!(stages & DMGS_STAGE_GEOMETRY)) {
-    if (K_cap == 0) return bfail(DMGS_EINVAL, "the blend stage needs a capacity");
-    dmgs::RenderArgs r = render_args(views, N, K_cap, total_tiles, tile_view, ws, L);
-    r.out_color = images;
-    dmgs::launch_render_fwd(r, st, variant);
-    return bcheck("batch render forward", st);
+  const Ctx c(b);
+  const size_t N = c.N;
+  const dmgs::MVArgs a = mv_args(b, c, params);
+  uint32_t* offs = at<uint32_t>(b->state, c.S.offsets);
+  if (stages & DMGS_STAGE_GEOMETRY) {
+    dmgs::launch_mv_preprocess(a, st);
+    uint32_t* gs = at<uint32_t>(b->state, c.S.gscratch);
+    dmgs::radix_sort_pairs(a.depth_keys, a.order, at<uint32_t>(b->state, c.S.keys_b),
+                           at<uint32_t>(b->state, c.S.order_b), N, 32, gs, st);
+    dmgs::launch_gather_u32(a.tiles, a.order, offs, (int)N, st);
+    dmgs::launch_inclusive_scan(offs, offs, N, gs, st);
+    if ((rc = bcheck("batch preprocess / depth sort / scan", st))) return rc;
   }
-  dmgs::launch_mv_preprocess(a, st);
-  if ((rc = bcheck("batch preprocess", st))) return rc;
-  uint32_t* gs = at<uint32_t>(ws, L.gscratch);
-  dmgs::radix_sort_pairs(a.depth_keys, a.order, at<uint32_t>(ws, L.keys_b), at<uint32_t>(ws, L.order_b), N, 32, gs,
-                         st);
-  uint32_t* offs = at<uint32_t>(ws, L.offsets);
-  dmgs::launch_gather_u32(a.tiles, a.order, offs, (int)N, st);
-  dmgs::launch_inclusive_scan(offs, offs, N, gs, st);
-  if ((rc = bcheck("batch depth sort / scan", st))) return rc;
-  if (K_cap == 0) return DMGS_OK;  // count only: read it at dmgs_batched_offsets()[1]
-  const uint32_t* K_dev = offs + (N - 1);
-  dmgs::DupArgs d{};
-  d.P = (int)N;
-  d.order = a.order;
-  d.offsets = offs;
-  d.splats = a.splats;
-  d.views = a.views;
-  d.per_view = (uint32_t)P;
-  d.tile_keys = at<uint32_t>(ws, L.tkeys_a);
-  d.values = at<uint32_t>(ws, L.vals_a);
-  d.K = (uint32_t)K_cap;
-  d.K_dev = K_dev;
-  d.chunk_first = at<uint32_t>(ws, L.bscratch);
-  dmgs::launch_duplicate(d, st);
-  dmgs::radix_sort_pairs(d.tile_keys, d.values, at<uint32_t>(ws, L.tkeys_b), at<uint32_t>(ws, L.vals_b),
-                         (size_t)K_cap, bits_for((uint32_t)total_tiles), at<uint32_t>(ws, L.bscratch), st, K_dev);
-  dmgs::launch_tile_ranges(d.tile_keys, (size_t)K_cap, at<uint2>(ws, L.ranges), total_tiles, st, K_dev);
-  if ((rc = bcheck("batch binning", st))) return rc;
-  if (!(stages & DMGS_STAGE_BLEND)) return DMGS_OK;
-  dmgs::RenderArgs r = render_args(views, N, K_cap, total_tiles, tile_view, ws, L);
-  r.out_color = images;
-  dmgs::launch_render_fwd(r, st, variant);
-  return bcheck("batch render forward", st);
+  if (stages & DMGS_STAGE_BINNING) {
+    const uint32_t K = (uint32_t)b->capacity;
+    dmgs::DupArgs d{};
+    d.P = (int)N;
+    d.order = a.order;
+    d.offsets = offs;
+    d.splats = a.splats;
+    d.views = a.views;
+    d.per_view = (uint32_t)b->num_gaussians;
+    d.tile_keys = at<uint32_t>(b->lists, c.L.tkeys_a);
+    d.values = at<uint32_t>(b->lists, c.L.vals_a);
+    d.K = K;
+    d.K_dev = c.count(b);
+    d.chunk_first = at<uint32_t>(b->lists, c.L.scratch);
+    dmgs::launch_duplicate(d, st);
+    dmgs::radix_sort_pairs(d.tile_keys, d.values, at<uint32_t>(b->lists, c.L.tkeys_b),
+                           at<uint32_t>(b->lists, c.L.vals_b), K, bits_for((uint32_t)b->total_tiles),
+                           at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev);
+    dmgs::launch_tile_ranges(d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_tiles, st, d.K_dev);
+    if ((rc = bcheck("batch binning", st))) return rc;
+  }
+  if (stages & DMGS_STAGE_BLEND) {
+    dmgs::RenderArgs r = render_args(b, c);
+    r.out_color = images;
+    dmgs::launch_render_fwd(r, st, b->variant);
+    if ((rc = bcheck("batch render forward", st))) return rc;
+  }
+  return DMGS_OK;
 }
 
-int dmgs_batched_backward(const dmgs_view* views, int32_t B, int64_t P, int32_t sh_degree, float scale_modifier,
-                        const float* params, int64_t K_cap, int32_t total_tiles, int64_t total_pixels,
-                        const int32_t* tile_view, void* ws, const float* dL_dimages, float* grads, int32_t variant,
-                        int32_t stages, dmgs_stream_t stream) {
-  int rc = check_batch(B, P, K_cap, total_tiles, total_pixels);
+int dmgs_batched_backward(const dmgs_batch* b, const float* params, const float* dL_dimages, float* grads,
+                          int32_t stages, dmgs_stream_t stream) {
+  int rc = check_batch(b);
   if (rc) return rc;
-  if (!views || !params || !ws || !tile_view || !dL_dimages || !grads || K_cap == 0)
-    return bfail(DMGS_EINVAL, "views, params, ws, tile_view, dL_dimages, grads and a capacity are required");
-  if (sh_degree < 0 || sh_degree > 3) return bfail(DMGS_EINVAL, "sh_degree must be in [0, 3]");
-  if (stages <= 0 || (stages & ~DMGS_STAGES_ALL)) return bfail(DMGS_EINVAL, "invalid stage mask %d", stages);
+  if (!params || !b->bwd_ws || !b->lists || !b->tile_view || b->capacity == 0)
+    return bfail(DMGS_EINVAL, "params, lists, a capacity, the tile map and the backward workspace are required");
+  if (stages <= 0 || (stages & ~(DMGS_STAGE_GEOMETRY | DMGS_STAGE_BLEND)))
+    return bfail(DMGS_EINVAL, "invalid stage mask %d", stages);
+  if ((stages & DMGS_STAGE_BLEND) && !dL_dimages) return bfail(DMGS_EINVAL, "dL_dimages is required");
+  if ((stages & DMGS_STAGE_GEOMETRY) && !grads) return bfail(DMGS_EINVAL, "grads is required");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const size_t N = (size_t)B * P;
-  BatchLayout L(N, (size_t)K_cap, (size_t)total_tiles, (size_t)total_pixels);
+  const Ctx c(b);
+  dmgs::MVArgs a = mv_args(b, c, params);
   if (stages & DMGS_STAGE_BLEND) {
-    dmgs::RenderArgs r = render_args(views, N, K_cap, total_tiles, tile_view, ws, L);
+    dmgs::launch_mv_clear_gacc(a, st);
+    dmgs::RenderArgs r = render_args(b, c);
     r.dL_dpix = dL_dimages;
-    dmgs::launch_render_bwd(r, st, variant);
+    dmgs::launch_render_bwd(r, st, b->variant);
     if ((rc = bcheck("batch render backward", st))) return rc;
   }
-  if (!(stages & DMGS_STAGE_GEOMETRY)) return DMGS_OK;
-  dmgs::MVArgs a = mv_args(views, B, P, sh_degree, scale_modifier, params, ws, L);
-  a.grads = grads;
-  dmgs::launch_mv_geometry_bwd(a, st);
-  return bcheck("batch geometry backward", st);
+  if (stages & DMGS_STAGE_GEOMETRY) {
+    a.grads = grads;
+    dmgs::launch_mv_geometry_bwd(a, st);
+    if ((rc = bcheck("batch geometry backward", st))) return rc;
+  }
+  return DMGS_OK;
 }
 
 }  // extern "C"

@@ -83,14 +83,20 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
     a.radii[i] = o.radius;
     a.tiles[i] = (uint32_t)o.tiles;
     a.depth_keys[i] = __float_as_uint(o.depth);
-    if (a.gacc) {  // clear the backward accumulator of every visible (view, Gaussian)
-      float4* gz = reinterpret_cast<float4*>(a.gacc + i * DMGS_GACC_STRIDE);
-      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-      gz[0] = z;
-      gz[1] = z;
-      gz[2] = z;
-    }
   }
+}
+
+// Zero the blend-backward accumulator of every visible (view, Gaussian)
+// entry; the blend backward only adds into entries that appear in the lists.
+__global__ __launch_bounds__(256) void mv_clear_gacc_kernel(const int32_t* __restrict__ radii,
+                                                            float* __restrict__ gacc, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || radii[i] <= 0) return;
+  float4* gz = reinterpret_cast<float4*>(gacc + i * DMGS_GACC_STRIDE);
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  gz[0] = z;
+  gz[1] = z;
+  gz[2] = z;
 }
 
 __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
@@ -164,6 +170,12 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s) {
   if (a.P <= 0 || a.B <= 0) return;
   hipLaunchKernelGGL(mv_preprocess_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+}
+
+void launch_mv_clear_gacc(const MVArgs& a, hipStream_t s) {
+  const size_t n = (size_t)a.B * a.P;
+  if (n == 0) return;
+  hipLaunchKernelGGL(mv_clear_gacc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.radii, a.gacc, n);
 }
 
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s) {

@@ -243,38 +243,57 @@ typedef struct dmgs_view {
 int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles, int64_t* total_pixels);
 /* Host-side: tile_view[t] = view owning global tile t (host array of total_tiles). */
 int dmgs_batched_tile_view(const dmgs_view* views, int32_t B, int32_t* tile_view, int32_t total_tiles);
-/* Workspace for B views, P Gaussians and room for K_cap (Gaussian, tile)
- * instances over the whole batch. */
-size_t dmgs_batched_workspace_size(int32_t B, int64_t P, int64_t K_cap, int32_t total_tiles, int64_t total_pixels);
-/* Byte offsets inside the workspace: [0] radii (B*P int32, view-major),
- * [1] instance count of the last forward (uint32), [2] tile ranges (uint2 per
- * global tile), [3] final_T, [4] n_contrib (per packed pixel), [5] splat records. */
-int dmgs_batched_offsets(int32_t B, int64_t P, int64_t K_cap, int32_t total_tiles, int64_t total_pixels,
-                         size_t offsets[6]);
-/* Stage mask of the batched calls: the per-Gaussian stages (forward:
- * preprocess, depth sort, scan, duplicate, tile sort, ranges; backward: the
- * per-Gaussian backward) and the blend.  Callers normally pass
- * DMGS_STAGES_ALL; a split lets a profiler bracket the blend on its own. */
+
+/* Workspaces of a batch (device memory, caller-owned):
+ *   state   K-independent: per (view, Gaussian) splats, keys, radii, scan;
+ *           per-tile ranges; per-pixel blend state.  Kept until backward.
+ *   lists   the (Gaussian, tile) instance lists for up to `capacity`
+ *           instances and their sort scratch.  Kept until backward.
+ *   bwd_ws  per (view, Gaussian) gradient accumulator (backward only). */
+size_t dmgs_batched_state_size(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels);
+size_t dmgs_batched_lists_size(int64_t capacity);
+size_t dmgs_batched_backward_size(int32_t B, int64_t P);
+/* Byte offsets inside `state`: [0] radii (B*P int32, view-major), [1] instance
+ * count of the last forward (uint32), [2] tile ranges (uint2 per global tile),
+ * [3] final_T, [4] n_contrib (per packed pixel), [5] splat records. */
+int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, size_t offsets[6]);
+
+typedef struct dmgs_batch {
+  const dmgs_view* views;   /* device copy of the B prepared records */
+  const int32_t* tile_view; /* device copy of the tile -> view map */
+  int32_t num_views;        /* B */
+  int32_t sh_degree;        /* active SH degree (coefficients are always 16 per colour) */
+  int64_t num_gaussians;    /* P */
+  float scale_modifier;
+  int32_t variant;          /* blend work-group shape 0..3 (same results; picked by an autotuner) */
+  int32_t total_tiles;
+  int32_t reserved;
+  int64_t total_pixels;
+  int64_t capacity;         /* instances the lists hold */
+  void* state;
+  void* lists;
+  void* bwd_ws;
+} dmgs_batch;
+
+/* Stage mask of the batched calls.  Forward: GEOMETRY = preprocess, depth
+ * sort and the scan of tile counts (afterwards the instance count is at
+ * offsets[1]); BINNING = duplicate, tile sort, ranges; BLEND = the blend.
+ * Backward: BLEND = blend backward into bwd_ws, GEOMETRY = the per-Gaussian
+ * backward into grads.  Stages run in that order within a call; a caller
+ * may split a call (e.g. read the count after GEOMETRY and size the lists,
+ * or bracket the blend for timing). */
 #define DMGS_STAGE_GEOMETRY 1
-#define DMGS_STAGE_BLEND 2
-#define DMGS_STAGES_ALL 3
-/* views / tile_view: device copies of the prepared records and of the tile
- * map.  With K_cap == 0 only preprocess, depth sort and the scan run, so the
- * instance count can be read to size K_cap.  When the count exceeds K_cap the
- * lists are truncated (the images are then wrong): callers read the count
- * back after the fact and re-plan.  variant selects the blend work-group
- * shape (0..3, same results; an autotuner picks one). */
-int dmgs_batched_forward(const dmgs_view* views, int32_t B, int64_t P, int32_t sh_degree, float scale_modifier,
-                         const float* params, int64_t K_cap, int32_t total_tiles, int64_t total_pixels,
-                         const int32_t* tile_view, void* ws, float* images, int32_t variant, int32_t stages,
+#define DMGS_STAGE_BINNING 2
+#define DMGS_STAGE_BLEND 4
+#define DMGS_STAGES_ALL 7
+/* When the count exceeds `capacity` the lists are truncated (the images are
+ * then wrong, nothing is accessed out of bounds): callers read the count
+ * back after the fact and re-plan. */
+int dmgs_batched_forward(const dmgs_batch* batch, const float* params, float* images, int32_t stages,
                          dmgs_stream_t stream);
 /* Gradients of the 59 raw channels (59 x P, overwritten) from dL/dimages
- * (packed like images), using the lists the forward left in ws.  The blend
- * stage accumulates per (view, Gaussian); the geometry stage reads that and
- * writes grads, so run the blend stage first when splitting. */
-int dmgs_batched_backward(const dmgs_view* views, int32_t B, int64_t P, int32_t sh_degree, float scale_modifier,
-                          const float* params, int64_t K_cap, int32_t total_tiles, int64_t total_pixels,
-                          const int32_t* tile_view, void* ws, const float* dL_dimages, float* grads, int32_t variant,
+ * (packed like images), using the state and lists the forward left. */
+int dmgs_batched_backward(const dmgs_batch* batch, const float* params, const float* dL_dimages, float* grads,
                           int32_t stages, dmgs_stream_t stream);
 
 #ifdef __cplusplus

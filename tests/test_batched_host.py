@@ -62,48 +62,64 @@ def test_prepare_rejects_bad_views():
 def test_workspace_and_offsets():
     lib = _lib()
     B, P, K, T, X = 3, 100000, 2_000_000, 7500, 1_920_000
-    n = lib.dmgs_batched_workspace_size(B, P, K, T, X)
-    # at least every compulsory buffer: 48 B splat + 7 x 4 B per entry, 4 x 4 B per instance,
-    # 8 B range per tile, 8 B per pixel, 64 B accumulator per entry
-    assert n >= B * P * (48 + 28 + 64) + K * 16 + T * 8 + X * 8
-    assert lib.dmgs_batched_workspace_size(0, P, K, T, X) == 0
-    assert lib.dmgs_batched_workspace_size(B, P, -1, T, X) == 0
+    n = lib.dmgs_batched_state_size(B, P, T, X)
+    # at least every compulsory buffer: 48 B splat + 7 x 4 B per entry, 8 B range per tile, 8 B per pixel
+    assert n >= B * P * (48 + 28) + T * 8 + X * 8
+    assert lib.dmgs_batched_lists_size(K) >= 16 * K
+    assert lib.dmgs_batched_backward_size(B, P) == B * P * 64
+    assert lib.dmgs_batched_state_size(0, P, T, X) == 0
+    assert lib.dmgs_batched_lists_size(0) == 0
     offs = (N.c_size_t * 6)()
-    N.check(lib.dmgs_batched_offsets(B, P, K, T, X, offs), "offsets")
+    N.check(lib.dmgs_batched_offsets(B, P, T, X, offs), "offsets")
     assert all(o % 4 == 0 and o < n for o in offs)
     assert len(set(offs)) == 6
-    # the count location does not depend on the capacity (counting pass uses K = 0)
-    offs0 = (N.c_size_t * 6)()
-    N.check(lib.dmgs_batched_offsets(B, P, 0, T, X, offs0), "offsets0")
-    assert offs0[0] == offs[0] and offs0[1] == offs[1]
+    assert lib.dmgs_batched_offsets(B, 0, T, X, offs) == -1
+
+
+def _desc(**kw):
+    d = N.DmgsBatch()
+    d.views = d.tile_view = d.state = d.lists = d.bwd_ws = 0x1000
+    d.num_views, d.sh_degree, d.num_gaussians = 2, 3, 100
+    d.scale_modifier, d.variant, d.total_tiles, d.total_pixels, d.capacity = 1.0, 0, 4, 100, 10
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return ctypes.byref(d)
+
+
+def test_batch_descriptor_layout():
+    # struct dmgs_batch: 2 pointers, 2 int32, int64, float, 3 int32, 2 int64, 3 pointers
+    assert ctypes.sizeof(N.DmgsBatch) == 88
+    assert N.DmgsBatch.capacity.offset == 56 and N.DmgsBatch.state.offset == 64
+    assert ctypes.sizeof(N.DmgsView) == 208
 
 
 def test_entry_points_validate_before_launch():
     lib = _lib()
     fake = ctypes.c_void_p(0x1000)
     A = BT.STAGES_ALL
-    # no views / params
-    assert lib.dmgs_batched_forward(None, 2, 100, 3, 1.0, None, 10, 4, 100, fake, fake, fake, 0, A, None) == -1
+    assert lib.dmgs_batched_forward(None, fake, fake, A, None) == -1
+    assert "batch is null" in N.last_error()
+    assert lib.dmgs_batched_forward(_desc(), None, fake, A, None) == -1
+    assert lib.dmgs_batched_forward(_desc(views=None), fake, fake, A, None) == -1
     # too many entries for 32-bit indices
-    assert lib.dmgs_batched_forward(fake, 4096, 1 << 20, 3, 1.0, fake, 10, 4, 100, fake, fake, fake, 0, A,
-                                    None) == -1
+    assert lib.dmgs_batched_forward(_desc(num_views=4096, num_gaussians=1 << 20), fake, fake, A, None) == -1
     assert "invalid batch" in N.last_error()
-    assert lib.dmgs_batched_forward(fake, 2, 100, 4, 1.0, fake, 10, 4, 100, fake, fake, fake, 0, A, None) == -1
+    assert lib.dmgs_batched_forward(_desc(sh_degree=4), fake, fake, A, None) == -1
     assert "sh_degree" in N.last_error()
-    for bad in (0, 4, -1):
-        assert lib.dmgs_batched_forward(fake, 2, 100, 3, 1.0, fake, 10, 4, 100, fake, fake, fake, 0, bad,
-                                        None) == -1
+    for bad in (0, 8, -1):
+        assert lib.dmgs_batched_forward(_desc(), fake, fake, bad, None) == -1
         assert "stage mask" in N.last_error()
-    # the blend alone needs a capacity
-    assert lib.dmgs_batched_forward(fake, 2, 100, 3, 1.0, fake, 0, 4, 100, fake, fake, fake, 0, BT.STAGE_BLEND,
-                                    None) == -1
-    # backward needs a capacity and every buffer
-    assert lib.dmgs_batched_backward(fake, 2, 100, 3, 1.0, fake, 0, 4, 100, fake, fake, fake, fake, 0, A,
-                                     None) == -1
-    assert lib.dmgs_batched_backward(fake, 2, 100, 3, 1.0, fake, 10, 4, 100, fake, fake, None, fake, 0, A,
-                                     None) == -1
-    assert lib.dmgs_batched_backward(fake, 2, 100, 3, 1.0, fake, 10, 4, 100, fake, fake, fake, fake, 0, 8,
-                                     None) == -1
+    # binning / blend need lists and a capacity; the blend needs images
+    assert lib.dmgs_batched_forward(_desc(capacity=0), fake, fake, BT.STAGE_BINNING, None) == -1
+    assert lib.dmgs_batched_forward(_desc(lists=None), fake, fake, BT.STAGE_BLEND, None) == -1
+    assert lib.dmgs_batched_forward(_desc(), fake, None, BT.STAGE_BLEND, None) == -1
+    # backward needs the accumulator, a capacity and its buffers
+    B2 = BT.STAGE_BLEND | BT.STAGE_GEOMETRY
+    assert lib.dmgs_batched_backward(_desc(bwd_ws=None), fake, fake, fake, B2, None) == -1
+    assert lib.dmgs_batched_backward(_desc(capacity=0), fake, fake, fake, B2, None) == -1
+    assert lib.dmgs_batched_backward(_desc(), fake, None, fake, B2, None) == -1
+    assert lib.dmgs_batched_backward(_desc(), fake, fake, None, B2, None) == -1
+    assert lib.dmgs_batched_backward(_desc(), fake, fake, fake, BT.STAGE_BINNING, None) == -1
 
 
 def test_pack_unpack_round_trip():
