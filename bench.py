@@ -55,6 +55,9 @@ def parse():
                     help="time budget of the sampled CPU-oracle baseline (0 disables)")
     ap.add_argument("--render-samples", type=int, default=50, help="forward-only renders for p50/p99")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-view-per-launch comparison")
+    ap.add_argument("--serving-requests", type=int, default=200,
+                    help="forward render requests for the serving figures (0 disables)")
+    ap.add_argument("--serving-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
     return ap.parse_args()
 
 
@@ -158,6 +161,27 @@ def run_plain(model, cams, targets, bg, steps, warmup):
     B = len(cams)
     return {"value": round(B * steps / elapsed, 3), "unit": "it/s", "ms_per_view": round(1000 * elapsed / (B * steps), 4),
             "stage_ms": {k: round(v, 4) for k, v in timer.averages_ms().items()}}
+
+
+def serving_run(n_requests, rate):
+    """Four resident scenes (0.5M / 1M / 2M / 3M Gaussians, same
+    distributions), Poisson arrivals at ``rate``, random scene, camera on a
+    sphere of radius U[3, 6], resolution drawn from 256^2 .. 1920x1080;
+    continuous batching over 4 streams with HBM-budget admission."""
+    from dmgs_amd import batched as BT
+    from dmgs_amd import scenes as S
+    from dmgs_amd import serving as SV
+    scenes = {}
+    for i, n in enumerate((500_000, 1_000_000, 2_000_000, 3_000_000)):
+        scenes[f"s{n}"] = BT.pack_model(S.free_scene(n, seed=10 + i, layout="mixture", shell_radius=3.0,
+                                                     device="cuda"))
+    names = sorted(scenes)
+    srv = SV.RenderServer(scenes, max_views=8, num_streams=4)
+    SV.serve_once(srv, SV.poisson_requests(names, 16, None, seed=99))  # warm-up
+    stats = SV.serve_once(srv, SV.poisson_requests(names, n_requests, rate, seed=3))
+    stats.update(rate_per_s=rate, requests=n_requests, budget_GB=round(srv.budget / 1e9, 1),
+                 peak_reserved_GB=round(srv.peak_reserved / 1e9, 3), max_in_flight=srv.max_active)
+    return stats
 
 
 def plain_latency(model, cam, bg, samples):
@@ -359,6 +383,12 @@ def main():
         "step_algorithmic_GBs": round(step_bytes / (ms_step * 1e-3) / 1e9, 2),
     }
     log(f"batched {it_s:.2f} it/s; plain {plain['value'] if plain else 'skipped'}; stages {stage_ms}")
+    if args.serving_requests > 0:
+        log(f"serving: {args.serving_requests} requests at {args.serving_rate}/s over 4 resident scenes")
+        del step, loss, r
+        torch.cuda.empty_cache()
+        result["serving"] = serving_run(args.serving_requests, args.serving_rate)
+        log(f"serving {result['serving']}")
     if args.cpu_baseline_seconds > 0:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(model, cams[0], args.cpu_baseline_seconds)

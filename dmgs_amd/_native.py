@@ -31,7 +31,7 @@ class DmgsView(ctypes.Structure):
         ("viewmatrix", c_float * 16), ("projmatrix", c_float * 16), ("campos", c_float * 4), ("bg", c_float * 4),
         ("tanfovx", c_float), ("tanfovy", c_float), ("focal_x", c_float), ("focal_y", c_float),
         ("width", c_int32), ("height", c_int32), ("grid_x", c_int32), ("grid_y", c_int32),
-        ("tile_base", c_int32), ("reserved", c_int32), ("pixel_base", c_int64),
+        ("tile_base", c_int32), ("cell_base", c_int32), ("pixel_base", c_int64),
     ]
 
 
@@ -40,7 +40,7 @@ class DmgsBatch(ctypes.Structure):
     _fields_ = [
         ("views", c_void_p), ("tile_view", c_void_p), ("num_views", c_int32), ("sh_degree", c_int32),
         ("num_gaussians", c_int64), ("scale_modifier", c_float), ("variant", c_int32), ("total_tiles", c_int32),
-        ("reserved", c_int32), ("total_pixels", c_int64), ("capacity", c_int64),
+        ("total_cells", c_int32), ("total_pixels", c_int64), ("capacity", c_int64),
         ("state", c_void_p), ("lists", c_void_p), ("bwd_ws", c_void_p),
     ]
 
@@ -82,12 +82,12 @@ _SIGNATURES = {
     "dmgs_scan_scratch_size": (c_size_t, [c_int64]),
     "dmgs_inclusive_scan_u32": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "dmgs_batched_prepare_views": (c_int32, [ctypes.POINTER(DmgsView), c_int32, ctypes.POINTER(c_int32),
-                                             ctypes.POINTER(c_int64)]),
+                                             ctypes.POINTER(c_int32), ctypes.POINTER(c_int64)]),
     "dmgs_batched_tile_view": (c_int32, [ctypes.POINTER(DmgsView), c_int32, c_void_p, c_int32]),
     "dmgs_batched_state_size": (c_size_t, [c_int32, c_int64, c_int32, c_int64]),
     "dmgs_batched_lists_size": (c_size_t, [c_int64]),
     "dmgs_batched_backward_size": (c_size_t, [c_int32, c_int64]),
-    "dmgs_batched_offsets": (c_int32, [c_int32, c_int64, c_int32, c_int64, ctypes.POINTER(c_size_t)]),
+    "dmgs_batched_offsets": (c_int32, [c_int32, c_int64, c_int32, c_int64, c_int64, ctypes.POINTER(c_size_t)]),
     "dmgs_batched_forward": (c_int32, [ctypes.POINTER(DmgsBatch), c_void_p, c_void_p, c_int32, c_void_p]),
     "dmgs_batched_backward": (c_int32, [ctypes.POINTER(DmgsBatch), c_void_p, c_void_p, c_void_p, c_int32,
                                         c_void_p]),

@@ -112,10 +112,10 @@ class BatchedRenderer:
         self.headroom, self.variant = float(headroom), int(variant)
         bgs = bg if isinstance(bg, (list, tuple)) else [bg] * self.B
         self.views = (N.DmgsView * self.B)(*[_view_record(c, b) for c, b in zip(cameras, bgs)])
-        tt, tp = N.c_int32(0), N.c_int64(0)
-        N.check(self.lib.dmgs_batched_prepare_views(self.views, self.B, ctypes.byref(tt), ctypes.byref(tp)),
-                "dmgs_batched_prepare_views")
-        self.total_tiles, self.total_pixels = tt.value, tp.value
+        tt, tc, tp = N.c_int32(0), N.c_int32(0), N.c_int64(0)
+        N.check(self.lib.dmgs_batched_prepare_views(self.views, self.B, ctypes.byref(tt), ctypes.byref(tc),
+                                                    ctypes.byref(tp)), "dmgs_batched_prepare_views")
+        self.total_tiles, self.total_cells, self.total_pixels = tt.value, tc.value, tp.value
         tv = torch.empty(self.total_tiles, dtype=torch.int32)
         N.check(self.lib.dmgs_batched_tile_view(self.views, self.B, ctypes.c_void_p(tv.data_ptr()),
                                                 self.total_tiles), "dmgs_batched_tile_view")
@@ -133,8 +133,8 @@ class BatchedRenderer:
         n = self.lib.dmgs_batched_state_size(self.B, self.P, self.total_tiles, self.total_pixels)
         if n == 0:
             raise RuntimeError("dmgs_batched_state_size rejected the batch")
-        offs = (N.c_size_t * 6)()
-        N.check(self.lib.dmgs_batched_offsets(self.B, self.P, self.total_tiles, self.total_pixels, offs),
+        offs = (N.c_size_t * 8)()
+        N.check(self.lib.dmgs_batched_offsets(self.B, self.P, self.total_tiles, self.total_pixels, 0, offs),
                 "dmgs_batched_offsets")
         self.state = torch.empty(n, dtype=torch.uint8, device=self.device)
         self._offsets = list(offs)
@@ -144,7 +144,7 @@ class BatchedRenderer:
         d.views, d.tile_view = self.views_dev.data_ptr(), self.tile_view.data_ptr()
         d.num_views, d.sh_degree, d.num_gaussians = self.B, self.sh_degree, self.P
         d.scale_modifier, d.variant = self.scale_modifier, self.variant
-        d.total_tiles, d.total_pixels = self.total_tiles, self.total_pixels
+        d.total_tiles, d.total_cells, d.total_pixels = self.total_tiles, self.total_cells, self.total_pixels
         d.capacity = self.K_cap if capacity is None else capacity
         d.state = self.state.data_ptr()
         d.lists = None if self.lists is None else self.lists.data_ptr()
@@ -245,6 +245,19 @@ class BatchedRenderer:
         """True when the last forward needed more instances than planned
         (its lists were truncated; re-plan and render again).  Reads back."""
         return (int(self.instance_count().item()) & 0xFFFFFFFF) > self.K_cap
+
+    def cell_lists(self):
+        """(ranges (cells, 2), sorted cell ids, sorted values) of the last
+        forward as CPU tensors; a value is entry id | tile mask << 28."""
+        offs = (N.c_size_t * 8)()
+        N.check(self.lib.dmgs_batched_offsets(self.B, self.P, self.total_tiles, self.total_pixels, self.K_cap, offs),
+                "dmgs_batched_offsets")
+        K = min(int(self.instance_count().item()) & 0xFFFFFFFF, self.K_cap)
+        o = self._offsets[2]
+        ranges = self.state[o:o + 8 * self.total_cells].view(torch.int32).view(-1, 2).cpu()
+        keys = self.lists[offs[6]:offs[6] + 4 * K].view(torch.int32).cpu()
+        vals = self.lists[offs[7]:offs[7] + 4 * K].view(torch.int32).cpu()
+        return ranges, keys, vals
 
     def radii(self):
         """(B, P) int32 radii of the last forward."""

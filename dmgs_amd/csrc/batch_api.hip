@@ -32,6 +32,7 @@ static_assert(offsetof(dmgs_view, tanfovx) == offsetof(dmgs::ViewDesc, tanfovx),
 static_assert(offsetof(dmgs_view, width) == offsetof(dmgs::ViewDesc, W), "dmgs_view layout");
 static_assert(offsetof(dmgs_view, bg) == offsetof(dmgs::ViewDesc, bg), "dmgs_view layout");
 static_assert(sizeof(dmgs_batch) == 88 && offsetof(dmgs_batch, capacity) == 56, "dmgs_batch layout");
+static_assert(offsetof(dmgs_view, cell_base) == offsetof(dmgs::ViewDesc, cell_base), "dmgs_view layout");
 
 namespace {
 
@@ -103,9 +104,12 @@ struct ListLayout {
 
 int check_batch(const dmgs_batch* b) {
   if (!b) return bfail(DMGS_EINVAL, "batch is null");
-  if (b->num_views <= 0 || b->num_gaussians <= 0 || (int64_t)b->num_views * b->num_gaussians > (int64_t)0x7fffffff)
-    return bfail(DMGS_EINVAL, "invalid batch: %d views x %lld Gaussians", b->num_views,
+  // entry ids share 32 bits with the 4-bit tile mask of the cell lists
+  if (b->num_views <= 0 || b->num_gaussians <= 0 ||
+      (int64_t)b->num_views * b->num_gaussians > (int64_t)DMGS_CELL_ID_MASK + 1)
+    return bfail(DMGS_EINVAL, "invalid batch: %d views x %lld Gaussians (at most 2^28 entries)", b->num_views,
                  (long long)b->num_gaussians);
+  if (b->total_cells <= 0 || b->total_cells > b->total_tiles) return bfail(DMGS_EINVAL, "invalid cell total");
   if (b->capacity < 0 || b->capacity > (int64_t)0xffffffff) return bfail(DMGS_EINVAL, "invalid instance capacity");
   if (b->total_tiles <= 0 || b->total_pixels <= 0) return bfail(DMGS_EINVAL, "invalid view totals");
   if (b->sh_degree < 0 || b->sh_degree > 3) return bfail(DMGS_EINVAL, "sh_degree must be in [0, 3]");
@@ -161,9 +165,10 @@ dmgs::RenderArgs render_args(const dmgs_batch* b, const Ctx& c) {
 
 extern "C" {
 
-int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles, int64_t* total_pixels) {
-  if (!views || B <= 0 || !total_tiles || !total_pixels) return bfail(DMGS_EINVAL, "bad arguments");
-  int64_t tiles = 0, pixels = 0;
+int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles, int32_t* total_cells,
+                               int64_t* total_pixels) {
+  if (!views || B <= 0 || !total_tiles || !total_cells || !total_pixels) return bfail(DMGS_EINVAL, "bad arguments");
+  int64_t tiles = 0, cells = 0, pixels = 0;
   for (int v = 0; v < B; ++v) {
     dmgs_view& d = views[v];
     if (d.width <= 0 || d.height <= 0 || !(d.tanfovx > 0.f) || !(d.tanfovy > 0.f))
@@ -173,12 +178,15 @@ int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles
     d.grid_x = (d.width + DMGS_BLOCK_X - 1) / DMGS_BLOCK_X;
     d.grid_y = (d.height + DMGS_BLOCK_Y - 1) / DMGS_BLOCK_Y;
     d.tile_base = (int32_t)tiles;
+    d.cell_base = (int32_t)cells;
     d.pixel_base = pixels;
     tiles += (int64_t)d.grid_x * d.grid_y;
+    cells += (int64_t)((d.grid_x + 1) / 2) * ((d.grid_y + 1) / 2);
     pixels += (int64_t)d.width * d.height;
     if (tiles > (int64_t)0x7fffffff) return bfail(DMGS_EINVAL, "too many tiles");
   }
   *total_tiles = (int32_t)tiles;
+  *total_cells = (int32_t)cells;
   *total_pixels = pixels;
   return DMGS_OK;
 }
@@ -209,12 +217,17 @@ size_t dmgs_batched_backward_size(int32_t B, int64_t P) {
   return (size_t)B * P * DMGS_GACC_STRIDE * 4;
 }
 
-int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, size_t offsets[6]) {
-  if (B <= 0 || P <= 0 || total_tiles <= 0 || total_pixels <= 0 || (int64_t)B * P > (int64_t)0x7fffffff)
+int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, int64_t capacity,
+                         size_t offsets[8]) {
+  if (B <= 0 || P <= 0 || total_tiles <= 0 || total_pixels <= 0 || capacity < 0 ||
+      (int64_t)B * P > (int64_t)DMGS_CELL_ID_MASK + 1)
     return bfail(DMGS_EINVAL, "invalid batch sizes");
   if (!offsets) return bfail(DMGS_EINVAL, "offsets is null");
   const size_t N = (size_t)B * P;
   StateLayout S(N, (size_t)total_tiles, (size_t)total_pixels);
+  ListLayout Ls((size_t)capacity);
+  offsets[6] = Ls.tkeys_a;
+  offsets[7] = Ls.vals_a;
   offsets[0] = S.radii;                  // (B*P) int32 radii, view-major
   offsets[1] = S.offsets + 4 * (N - 1);  // uint32 instance count of the batch
   offsets[2] = S.ranges;
@@ -263,9 +276,9 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     d.chunk_first = at<uint32_t>(b->lists, c.L.scratch);
     dmgs::launch_duplicate(d, st);
     dmgs::radix_sort_pairs(d.tile_keys, d.values, at<uint32_t>(b->lists, c.L.tkeys_b),
-                           at<uint32_t>(b->lists, c.L.vals_b), K, bits_for((uint32_t)b->total_tiles),
+                           at<uint32_t>(b->lists, c.L.vals_b), K, bits_for((uint32_t)b->total_cells),
                            at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev);
-    dmgs::launch_tile_ranges(d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_tiles, st, d.K_dev);
+    dmgs::launch_tile_ranges(d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells, st, d.K_dev);
     if ((rc = bcheck("batch binning", st))) return rc;
   }
   if (stages & DMGS_STAGE_BLEND) {

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
     sp[1] = make_float4(o.conic.z, G.o, rgb.x, rgb.y);
     sp[2] = make_float4(rgb.z, o.depth, __int_as_float(o.radius), 0.0f);
     a.radii[i] = o.radius;
-    a.tiles[i] = (uint32_t)o.tiles;
+    a.tiles[i] = (uint32_t)cell_count(o.rmin, o.rmax);  // instances are per binning cell
     a.depth_keys[i] = __float_as_uint(o.depth);
   }
 }

@@ -42,7 +42,7 @@ struct ViewDesc {
   float tanfovx, tanfovy, focal_x, focal_y;
   int32_t W, H, grid_x, grid_y;
   int32_t tile_base;    // first global tile id of the view
-  int32_t pad;
+  int32_t cell_base;    // first global binning-cell id of the view (multi-view lists)
   int64_t pixel_base;   // first pixel of the view in the packed per-pixel buffers
 };
 
@@ -71,6 +71,30 @@ __device__ __forceinline__ void tile_rect(float2 p, int r, int gx, int gy, int2&
   rmin.y = min(gy, max(0, (int)((p.y - fr) / b)));
   rmax.x = min(gx, max(0, (int)((((p.x + fr) + b) - 1.0f) / b)));
   rmax.y = min(gy, max(0, (int)((((p.y + fr) + b) - 1.0f) / b)));
+}
+
+// --- binning cells of the multi-view lists --------------------------------
+// The multi-view path bins each splat into cells of 2 x 2 blend tiles and
+// tags every (splat, cell) instance with a 4-bit mask of the cell's tiles the
+// splat's tile rectangle covers (bit (ty & 1) * 2 + (tx & 1)).  A tile's
+// list is its cell's list filtered by its bit, in the same order, so it
+// equals the per-tile list of the reference binning.
+#define DMGS_CELL_MASK_SHIFT 28
+#define DMGS_CELL_ID_MASK 0x0FFFFFFFu
+__device__ __forceinline__ int cells_x(int gx) { return (gx + 1) >> 1; }
+// Number of cells a non-empty tile rectangle [rmin, rmax) touches.
+__device__ __forceinline__ int cell_count(int2 rmin, int2 rmax) {
+  return (((rmax.x - 1) >> 1) - (rmin.x >> 1) + 1) * (((rmax.y - 1) >> 1) - (rmin.y >> 1) + 1);
+}
+// Mask of the tiles of cell (cx, cy) inside the tile rectangle [rmin, rmax).
+__device__ __forceinline__ uint32_t cell_mask(int2 rmin, int2 rmax, int cx, int cy) {
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int tx = 2 * cx + (q & 1), ty = 2 * cy + (q >> 1);
+    if (tx >= rmin.x && tx < rmax.x && ty >= rmin.y && ty < rmax.y) m |= 1u << q;
+  }
+  return m;
 }
 
 // --- wave64 reductions --------------------------------------------------

@@ -52,8 +52,9 @@ void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, si
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int n, hipStream_t s);
 
 // Emit (tile, splat) pairs in depth order.  Single view: grid_x/grid_y give
-// the tile grid.  Several views: entry g belongs to view g / per_view and
-// tiles are numbered globally from that view's tile_base.
+// the tile grid.  Several views: entry g belongs to view g / per_view, the
+// pairs are (binning cell, splat | tile mask << 28) with cells numbered
+// globally from that view's cell_base.
 struct DupArgs {
   int P;                        // depth-sorted entries
   const uint32_t* order;

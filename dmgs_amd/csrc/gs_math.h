@@ -109,6 +109,7 @@ struct SplatGeom {
   float depth;
   int radius;
   int tiles;
+  int2 rmin, rmax;  // tile rectangle [rmin, rmax)
 };
 
 // Cull, project, EWA cov2D (+0.3), conic, radius and tile rectangle for one
@@ -147,7 +148,8 @@ __device__ __forceinline__ bool project_splat(float3 p, const float c[6], const 
   const float lambda1 = mid + disc, lambda2 = mid - disc;
   const float radius = ceilf(3.f * sqrtf(fmaxf(lambda1, lambda2)));
   o.pix = make_float2(ndc_to_pix(ph.x * pw, v.W), ndc_to_pix(ph.y * pw, v.H));
-  int2 rmin, rmax;
+  int2& rmin = o.rmin;
+  int2& rmax = o.rmax;
   tile_rect(o.pix, (int)radius, v.gx, v.gy, rmin, rmax);
   o.tiles = (rmax.x - rmin.x) * (rmax.y - rmin.y);
   if (o.tiles == 0) return false;

@@ -62,7 +62,8 @@ struct TileCtx {
   size_t plane;        // W * H of the view
   size_t img;          // offset of the view's colour planes
   float bg0, bg1, bg2;
-  uint2 range;
+  uint32_t bit;        // multi-view: this tile's bit in its cell's instance tags
+  uint2 range;         // single view: the tile's list; multi-view: its cell's list
 };
 
 template <int SUB>
@@ -73,10 +74,15 @@ __device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
   const int q = blockIdx.x - c.tile * Q * Q;
   int lt = c.tile, gx = a.grid_x;
   int64_t pbase = 0;
+  int list = c.tile;
+  c.bit = 0;
   if (a.views) {
     const ViewDesc& v = a.views[a.tile_view[c.tile]];
     lt = c.tile - v.tile_base;
     gx = v.grid_x;
+    const int ltx = lt % gx, lty = lt / gx;
+    list = v.cell_base + (lty >> 1) * cells_x(gx) + (ltx >> 1);
+    c.bit = 1u << (((lty & 1) << 1) | (ltx & 1));
     c.W = v.W;
     c.H = v.H;
     c.bg0 = v.bg[0];
@@ -100,11 +106,51 @@ __device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
   c.img = 3 * (size_t)pbase;
   uint32_t K = a.K;
   if (a.K_dev) K = min(K, *a.K_dev);
-  uint2 r = a.ranges[c.tile];
+  uint2 r = a.ranges[list];
   r.y = min(r.y, K);
   r.x = min(r.x, r.y);
   c.range = r;
   return c;
+}
+
+// Stage one batch of a cell list (multi-view): visit BATCH list positions —
+// forward: base, base+1, ... below len; backward: top-1, top-2, ... down to 0
+// — keep the instances tagged with this tile's bit and stage them in visit
+// order.  s_pos receives each staged instance's list position.  Returns the
+// number staged (workgroup-uniform); ends with a barrier.
+template <int NT, int BATCH, bool BACKWARD>
+__device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_pos, uint32_t* s_wc,
+                                                const RenderArgs& a, const TileCtx& c, int start, int limit) {
+  constexpr int NW = NT / DMGS_WAVE;
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = lanemask_lt();
+  int run = 0;
+#pragma unroll
+  for (int k = 0; k < (BATCH + NT - 1) / NT; ++k) {
+    const int s = k * NT + (int)threadIdx.x;
+    const int q = BACKWARD ? start - 1 - s : start + s;
+    const bool valid = s < BATCH && (BACKWARD ? q >= 0 : q < limit);
+    const uint32_t val = valid ? a.point_list[c.range.x + q] : 0u;
+    const bool member = valid && ((val >> DMGS_CELL_MASK_SHIFT) & c.bit);
+    const uint64_t bal = __ballot(member);
+    if (lane == 0) s_wc[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    int before = run, total = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const int cnt = (int)s_wc[w];
+      before += (w < (int)wave) ? cnt : 0;
+      total += cnt;
+    }
+    if (member) {
+      const int slot = before + (int)__popcll(bal & lt);
+      stage_splat(L, slot, a.splats, val & DMGS_CELL_ID_MASK, a.P);
+      s_pos[slot] = (uint32_t)q;
+    }
+    run += total;
+    __syncthreads();
+  }
+  return run;
 }
 
 template <int SUB, int BATCH>
@@ -163,13 +209,71 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_kernel(RenderArgs a) {
   }
 }
 
+// Forward over cell lists (multi-view).  Same per-pixel arithmetic and order
+// as render_fwd_kernel; the per-pixel "last contributor" is one past the
+// list position of the last contributing instance.
 template <int SUB, int BATCH>
+__global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a) {
+  constexpr int NT = SUB * SUB;
+  __shared__ SplatLDS<BATCH> L;
+  __shared__ uint32_t s_pos[BATCH];
+  __shared__ uint32_t s_wc[NT / DMGS_WAVE];
+  const TileCtx c = tile_ctx<SUB>(a);
+  const float pxf = (float)c.px, pyf = (float)c.py;
+  const int len = (int)(c.range.y - c.range.x);
+
+  bool done = !c.inside;
+  float T = 1.0f;
+  float C0 = 0.f, C1 = 0.f, C2 = 0.f;
+  uint32_t last_contributor = 0;
+
+  for (int base = 0; base < len; base += BATCH) {
+    if (__syncthreads_count(done) == NT) break;
+    const int n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, base, len);
+    for (int j = 0; !done && j < n; ++j) {
+      const float2 xy = L.xy[j];
+      const float4 co = L.con_o[j];
+      const float dx = xy.x - pxf, dy = xy.y - pyf;
+      const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+      if (power > 0.0f) continue;
+      const float alpha = fminf(0.99f, co.w * __expf(power));
+      if (alpha < 1.0f / 255.0f) continue;
+      const float test_T = T * (1.0f - alpha);
+      if (test_T < 0.0001f) {
+        done = true;
+        continue;
+      }
+      const float4 col = L.rgb_id[j];
+      const float w = alpha * T;
+      C0 += col.x * w;
+      C1 += col.y * w;
+      C2 += col.z * w;
+      T = test_T;
+      last_contributor = s_pos[j] + 1;
+    }
+  }
+  if (c.inside) {
+    const size_t pid = c.pix - c.img / 3;
+    a.final_T[c.pix] = T;
+    a.n_contrib[c.pix] = last_contributor;
+    float* out = a.out_color + c.img;
+    out[pid] = C0 + T * c.bg0;
+    out[c.plane + pid] = C1 + T * c.bg1;
+    out[2 * c.plane + pid] = C2 + T * c.bg2;
+  }
+}
+
+// Backward.  CELLS: the lists are cell lists (multi-view); positions replace
+// the per-tile counter and the staging keeps only this tile's instances.
+template <int SUB, int BATCH, bool CELLS>
 __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
   constexpr int NT = SUB * SUB;
   constexpr int NW = NT / DMGS_WAVE;
   __shared__ SplatLDS<BATCH> L;
   __shared__ uint32_t s_max[NW];
   __shared__ float s_acc[BATCH][DMGS_GACC_USED];
+  __shared__ uint32_t s_pos[CELLS ? BATCH : 1];
+  __shared__ uint32_t s_wc[NW];
   const TileCtx c = tile_ctx<SUB>(a);
   const float pxf = (float)c.px, pyf = (float)c.py;
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -188,7 +292,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
   const float bg_dot = c.bg0 * g0 + c.bg1 * g1 + c.bg2 * g2;
   const float ddelx_dx = 0.5f * (float)c.W, ddely_dy = 0.5f * (float)c.H;
 
-  // Entries at list positions >= max(n_contrib) of the workgroup contribute nothing.
+  // Entries at list positions >= max(last contributor) of the workgroup contribute nothing.
   uint32_t m = last_contributor;
   for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
   if (lane == 0) s_max[wave] = m;
@@ -206,16 +310,29 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
   const unsigned row_lane = lane & 15;  // lane within its DPP row of 16
   for (int r = 0; r < rounds; ++r) {
     __syncthreads();
-    for (int s = threadIdx.x; s < BATCH; s += NT) {
-      const int pos = r * BATCH + s;
-      if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P);
+    int n;
+    if constexpr (CELLS) {
+      for (int s = threadIdx.x; s < BATCH; s += NT) {
 #pragma unroll
-      for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
+        for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
+      }
+      n = stage_cell_batch<NT, BATCH, true>(L, s_pos, s_wc, a, c, todo_total - r * BATCH, 0);
+    } else {
+      for (int s = threadIdx.x; s < BATCH; s += NT) {
+        const int pos = r * BATCH + s;
+        if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P);
+#pragma unroll
+        for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
+      }
+      __syncthreads();
+      n = min(BATCH, todo_total - r * BATCH);
     }
-    __syncthreads();
-    const int n = min(BATCH, todo_total - r * BATCH);
     for (int j = 0; j < n; ++j) {
-      --contributor;
+      if constexpr (CELLS) {
+        contributor = s_pos[j];
+      } else {
+        --contributor;
+      }
       float v[DMGS_GACC_USED];
 #pragma unroll
       for (int k = 0; k < DMGS_GACC_USED; ++k) v[k] = 0.f;
@@ -284,13 +401,20 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
 template <int SUB, int BATCH>
 static void launch_fwd_t(const RenderArgs& a, hipStream_t s) {
   constexpr int Q = DMGS_BLOCK_X / SUB;
-  hipLaunchKernelGGL((render_fwd_kernel<SUB, BATCH>), dim3(a.num_tiles * Q * Q), dim3(SUB * SUB), 0, s, a);
+  if (a.views)
+    hipLaunchKernelGGL((render_fwd_cells_kernel<SUB, BATCH>), dim3(a.num_tiles * Q * Q), dim3(SUB * SUB), 0, s, a);
+  else
+    hipLaunchKernelGGL((render_fwd_kernel<SUB, BATCH>), dim3(a.num_tiles * Q * Q), dim3(SUB * SUB), 0, s, a);
 }
 
 template <int SUB, int BATCH>
 static void launch_bwd_t(const RenderArgs& a, hipStream_t s) {
   constexpr int Q = DMGS_BLOCK_X / SUB;
-  hipLaunchKernelGGL((render_bwd_kernel<SUB, BATCH>), dim3(a.num_tiles * Q * Q), dim3(SUB * SUB), 0, s, a);
+  if (a.views)
+    hipLaunchKernelGGL((render_bwd_kernel<SUB, BATCH, true>), dim3(a.num_tiles * Q * Q), dim3(SUB * SUB), 0, s, a);
+  else
+    hipLaunchKernelGGL((render_bwd_kernel<SUB, BATCH, false>), dim3(a.num_tiles * Q * Q), dim3(SUB * SUB), 0, s,
+                       a);
 }
 
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {

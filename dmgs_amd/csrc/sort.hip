@@ -377,6 +377,7 @@ __global__ __launch_bounds__(256) void chunk_first_kernel(DupArgs a) {
 __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
   __shared__ uint32_t s_end[256], s_g[256], s_tb[256];
   __shared__ int s_x0[256], s_y0[256], s_w[256], s_gx[256];
+  __shared__ uint32_t s_rx[256], s_ry[256];  // multi-view: tile rectangle, [lo, hi) packed 16:16
   const uint32_t K = device_count(a.K, a.K_dev);
   const uint32_t lo_slot = blockIdx.x * kDupChunk;
   if (lo_slot >= K) return;  // uniform per block
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
   while (true) {
     // stage Gaussians first .. first+255 (those past P get an empty range)
     const uint32_t s = first + threadIdx.x;
-    uint32_t end = 0xFFFFFFFFu, g = 0, tb = 0;
+    uint32_t end = 0xFFFFFFFFu, g = 0, tb = 0, rx = 0, ry = 0;
     int x0 = 0, y0 = 0, w = 1, gx = a.grid_x;
     if (s < P) {
       g = a.order[s];
@@ -394,19 +395,28 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
       end = min(a.offsets[s], K);
       if (end > beg && g < P) {
         int gy = a.grid_y;
-        if (a.views) {
-          const ViewDesc& v = a.views[g / a.per_view];
-          gx = v.grid_x;
-          gy = v.grid_y;
-          tb = (uint32_t)v.tile_base;
+        const ViewDesc* v = a.views ? &a.views[g / a.per_view] : nullptr;
+        if (v) {
+          gx = v->grid_x;
+          gy = v->grid_y;
         }
         const float4 s0 = a.splats[(size_t)DMGS_SPLAT_F4 * g];
         const int r = __float_as_int(a.splats[(size_t)DMGS_SPLAT_F4 * g + 2].z);
         int2 rmin, rmax;
         tile_rect(make_float2(s0.x, s0.y), r, gx, gy, rmin, rmax);
-        x0 = rmin.x;
-        y0 = rmin.y;
-        w = max(1, rmax.x - rmin.x);
+        if (v) {  // emit per binning cell of 2x2 tiles
+          rx = (uint32_t)rmin.x | ((uint32_t)rmax.x << 16);
+          ry = (uint32_t)rmin.y | ((uint32_t)rmax.y << 16);
+          tb = (uint32_t)v->cell_base;
+          x0 = rmin.x >> 1;
+          y0 = rmin.y >> 1;
+          w = max(1, ((rmax.x - 1) >> 1) + 1 - x0);
+          gx = cells_x(gx);
+        } else {
+          x0 = rmin.x;
+          y0 = rmin.y;
+          w = max(1, rmax.x - rmin.x);
+        }
       }
     }
     s_end[threadIdx.x] = end;
@@ -416,6 +426,8 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
     s_y0[threadIdx.x] = y0;
     s_w[threadIdx.x] = w;
     s_gx[threadIdx.x] = gx;
+    s_rx[threadIdx.x] = rx;
+    s_ry[threadIdx.x] = ry;
     __syncthreads();
     const uint32_t win_begin = (first == 0) ? 0u : min(a.offsets[first - 1], K);
     const uint32_t win_end = min(s_end[255], hi_slot);
@@ -431,9 +443,16 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
       const uint32_t local = o - beg;
       const uint32_t wj = (uint32_t)s_w[lo];
       const uint32_t row = local / wj;
-      a.tile_keys[o] = s_tb[lo] + ((uint32_t)s_y0[lo] + row) * (uint32_t)s_gx[lo] + (uint32_t)s_x0[lo] +
-                       (local - row * wj);
-      a.values[o] = s_g[lo];
+      const uint32_t cx = (uint32_t)s_x0[lo] + (local - row * wj), cy = (uint32_t)s_y0[lo] + row;
+      a.tile_keys[o] = s_tb[lo] + cy * (uint32_t)s_gx[lo] + cx;
+      uint32_t val = s_g[lo];
+      if (a.views) {
+        const uint32_t rx = s_rx[lo], ry = s_ry[lo];
+        const int2 rmin = make_int2((int)(rx & 0xFFFFu), (int)(ry & 0xFFFFu));
+        const int2 rmax = make_int2((int)(rx >> 16), (int)(ry >> 16));
+        val |= cell_mask(rmin, rmax, (int)cx, (int)cy) << DMGS_CELL_MASK_SHIFT;
+      }
+      a.values[o] = val;
     }
     const bool more = win_end < hi_slot && first + 256 < P;
     __syncthreads();

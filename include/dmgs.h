@@ -234,13 +234,17 @@ typedef struct dmgs_view {
   int32_t width, height;
   int32_t grid_x, grid_y;   /* filled */
   int32_t tile_base;        /* filled */
-  int32_t reserved;
+  int32_t cell_base;        /* filled: first binning cell (2x2 tiles) of the view */
   int64_t pixel_base;       /* filled */
 } dmgs_view;
 
-/* Host-side: fill the derived fields of B host records and return the tile
- * and pixel totals of the batch. */
-int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles, int64_t* total_pixels);
+/* Host-side: fill the derived fields of B host records and return the tile,
+ * binning-cell and pixel totals of the batch.  The batched lists bin splats
+ * into cells of 2x2 tiles (32x32 pixels), each instance tagged with the
+ * cell's tiles it covers; a tile's list is its cell's list filtered by its
+ * tag, i.e. exactly the reference per-tile list. */
+int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles, int32_t* total_cells,
+                               int64_t* total_pixels);
 /* Host-side: tile_view[t] = view owning global tile t (host array of total_tiles). */
 int dmgs_batched_tile_view(const dmgs_view* views, int32_t B, int32_t* tile_view, int32_t total_tiles);
 
@@ -254,9 +258,13 @@ size_t dmgs_batched_state_size(int32_t B, int64_t P, int32_t total_tiles, int64_
 size_t dmgs_batched_lists_size(int64_t capacity);
 size_t dmgs_batched_backward_size(int32_t B, int64_t P);
 /* Byte offsets inside `state`: [0] radii (B*P int32, view-major), [1] instance
- * count of the last forward (uint32), [2] tile ranges (uint2 per global tile),
- * [3] final_T, [4] n_contrib (per packed pixel), [5] splat records. */
-int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, size_t offsets[6]);
+ * count of the last forward (uint32), [2] cell ranges (uint2 per global
+ * cell), [3] final_T, [4] last contributor (uint32 per packed pixel: one past
+ * the position of its last contributing instance in its cell's list),
+ * [5] splat records; inside `lists`: [6] sorted cell ids, [7] sorted values
+ * (entry id | tile mask << 28). */
+int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, int64_t capacity,
+                         size_t offsets[8]);
 
 typedef struct dmgs_batch {
   const dmgs_view* views;   /* device copy of the B prepared records */
@@ -267,9 +275,9 @@ typedef struct dmgs_batch {
   float scale_modifier;
   int32_t variant;          /* blend work-group shape 0..3 (same results; picked by an autotuner) */
   int32_t total_tiles;
-  int32_t reserved;
+  int32_t total_cells;
   int64_t total_pixels;
-  int64_t capacity;         /* instances the lists hold */
+  int64_t capacity;         /* (splat, cell) instances the lists hold */
   void* state;
   void* lists;
   void* bwd_ws;

@@ -29,18 +29,20 @@ def test_prepare_views_and_tile_map():
     lib = _lib()
     sizes = [(800, 800), (1920, 1080), (33, 17), (16, 16)]
     cams, views = _views(sizes)
-    tt, tp = N.c_int32(0), N.c_int64(0)
-    N.check(lib.dmgs_batched_prepare_views(views, len(sizes), ctypes.byref(tt), ctypes.byref(tp)), "prepare")
-    tiles = pixels = 0
+    tt, tc, tp = N.c_int32(0), N.c_int32(0), N.c_int64(0)
+    N.check(lib.dmgs_batched_prepare_views(views, len(sizes), ctypes.byref(tt), ctypes.byref(tc), ctypes.byref(tp)),
+            "prepare")
+    tiles = cells = pixels = 0
     for v, (w, h) in enumerate(sizes):
         d = views[v]
         assert (d.grid_x, d.grid_y) == ((w + 15) // 16, (h + 15) // 16)
-        assert d.tile_base == tiles and d.pixel_base == pixels
+        assert d.tile_base == tiles and d.pixel_base == pixels and d.cell_base == cells
         assert d.focal_x == pytest.approx(w / (2 * math.tan(cams[v].FoVx / 2)), rel=1e-6)
         assert d.focal_y == pytest.approx(h / (2 * math.tan(cams[v].FoVy / 2)), rel=1e-6)
         tiles += d.grid_x * d.grid_y
+        cells += ((w + 31) // 32) * ((h + 31) // 32)  # cells of 2x2 tiles = 32x32 pixels
         pixels += w * h
-    assert (tt.value, tp.value) == (tiles, pixels)
+    assert (tt.value, tc.value, tp.value) == (tiles, cells, pixels)
     tv = torch.full((tiles,), -1, dtype=torch.int32)
     N.check(lib.dmgs_batched_tile_view(views, len(sizes), ctypes.c_void_p(tv.data_ptr()), tiles), "tile_view")
     expect = torch.cat([torch.full((views[v].grid_x * views[v].grid_y,), v, dtype=torch.int32)
@@ -54,8 +56,8 @@ def test_prepare_rejects_bad_views():
     lib = _lib()
     _, views = _views([(64, 64), (32, 32)])
     views[1].width = 0
-    tt, tp = N.c_int32(0), N.c_int64(0)
-    assert lib.dmgs_batched_prepare_views(views, 2, ctypes.byref(tt), ctypes.byref(tp)) == -1
+    tt, tc, tp = N.c_int32(0), N.c_int32(0), N.c_int64(0)
+    assert lib.dmgs_batched_prepare_views(views, 2, ctypes.byref(tt), ctypes.byref(tc), ctypes.byref(tp)) == -1
     assert "view 1" in N.last_error()
 
 
@@ -69,11 +71,13 @@ def test_workspace_and_offsets():
     assert lib.dmgs_batched_backward_size(B, P) == B * P * 64
     assert lib.dmgs_batched_state_size(0, P, T, X) == 0
     assert lib.dmgs_batched_lists_size(0) == 0
-    offs = (N.c_size_t * 6)()
-    N.check(lib.dmgs_batched_offsets(B, P, T, X, offs), "offsets")
-    assert all(o % 4 == 0 and o < n for o in offs)
-    assert len(set(offs)) == 6
-    assert lib.dmgs_batched_offsets(B, 0, T, X, offs) == -1
+    offs = (N.c_size_t * 8)()
+    N.check(lib.dmgs_batched_offsets(B, P, T, X, K, offs), "offsets")
+    assert all(o % 4 == 0 and o < n for o in offs[:6])
+    assert len(set(offs[:6])) == 6
+    assert offs[6] < offs[7] < lib.dmgs_batched_lists_size(K)
+    assert lib.dmgs_batched_offsets(B, 0, T, X, K, offs) == -1
+    assert lib.dmgs_batched_offsets(B, 1 << 27, T, X, K, offs) == -1  # more than 2^28 entries
 
 
 def _desc(**kw):
@@ -81,6 +85,7 @@ def _desc(**kw):
     d.views = d.tile_view = d.state = d.lists = d.bwd_ws = 0x1000
     d.num_views, d.sh_degree, d.num_gaussians = 2, 3, 100
     d.scale_modifier, d.variant, d.total_tiles, d.total_pixels, d.capacity = 1.0, 0, 4, 100, 10
+    d.total_cells = 1
     for k, v in kw.items():
         setattr(d, k, v)
     return ctypes.byref(d)
@@ -90,6 +95,7 @@ def test_batch_descriptor_layout():
     # struct dmgs_batch: 2 pointers, 2 int32, int64, float, 3 int32, 2 int64, 3 pointers
     assert ctypes.sizeof(N.DmgsBatch) == 88
     assert N.DmgsBatch.capacity.offset == 56 and N.DmgsBatch.state.offset == 64
+    assert N.DmgsBatch.total_cells.offset == 44 and N.DmgsView.cell_base.offset == 196
     assert ctypes.sizeof(N.DmgsView) == 208
 
 
@@ -104,8 +110,11 @@ def test_entry_points_validate_before_launch():
     # too many entries for 32-bit indices
     assert lib.dmgs_batched_forward(_desc(num_views=4096, num_gaussians=1 << 20), fake, fake, A, None) == -1
     assert "invalid batch" in N.last_error()
+    assert lib.dmgs_batched_forward(_desc(num_views=2, num_gaussians=(1 << 27) + 1), fake, fake, A, None) == -1
     assert lib.dmgs_batched_forward(_desc(sh_degree=4), fake, fake, A, None) == -1
     assert "sh_degree" in N.last_error()
+    assert lib.dmgs_batched_forward(_desc(total_cells=0), fake, fake, A, None) == -1
+    assert lib.dmgs_batched_forward(_desc(total_cells=5), fake, fake, A, None) == -1  # more cells than tiles
     for bad in (0, 8, -1):
         assert lib.dmgs_batched_forward(_desc(), fake, fake, bad, None) == -1
         assert "stage mask" in N.last_error()

@@ -170,17 +170,24 @@ def test_instance_count_and_overflow():
     bgs = _bgs(2)
     r, params = _batched(model, cams, bgs)
     K = r.count_instances(params)
-    # sum of the single-view counts (binning of the drop-in path)
+    # the (splat, cell) instances of the drop-in path's tile rectangles: a
+    # rectangle [x0, x1) x [y0, y1) of 16-pixel tiles touches the cells
+    # x0 // 2 .. (x1 - 1) // 2 (and the same in y)
     from dmgs_amd import rasterizer as RZ
-    from tests.helpers import settings_from_cam
+    from tests.helpers import settings_from_cam, workspace_views
     m = model.to("cuda")
-    Ks = []
-    for cam, bg in zip(cams, bgs):
+    Ks, Ktiles = [], []
+    for (w, h), cam, bg in zip(sizes, cams, bgs):
         rs = settings_from_cam(cam, bg, 3)
-        _, _, st = RZ.forward_raw(m.get_xyz, m.get_opacity, m.get_features, None, m.get_scaling, m.get_rotation,
-                                  None, rs)
-        Ks.append(st["K"])
+        _, radii, st = RZ.forward_raw(m.get_xyz, m.get_opacity, m.get_features, None, m.get_scaling,
+                                      m.get_rotation, None, rs)
+        wv = workspace_views(st, w, h)
+        vis = radii.cpu().numpy() > 0
+        x0, y0, x1, y1 = OR.rect_f32(wv["xy"][vis], radii.cpu().numpy()[vis], w, h)
+        Ks.append(int((((x1 - 1) // 2 - x0 // 2 + 1) * ((y1 - 1) // 2 - y0 // 2 + 1)).sum()))
+        Ktiles.append(st["K"])
     assert K == sum(Ks)
+    assert K < sum(Ktiles)
     images_full = r.forward(params).clone()
     assert not r.overflowed()
     # a capacity below the count: reported, truncated, stays in bounds
@@ -251,3 +258,47 @@ def test_graph_replay_matches_eager():
         step.run()  # eager work between replays
         torch.cuda.synchronize()
     assert not r.overflowed()
+
+
+@pytest.mark.parametrize("sizes", [[(200, 150), (77, 53), (256, 256)], [(33, 17)]])
+def test_cell_lists_equal_reference_tile_lists(sizes):
+    """Every tile's list, read from its binning cell's list filtered by the
+    tile's tag bit, equals the reference (tile | depth)-sorted list of that
+    tile (oracle binning of the same FP32 splat positions, radii and
+    depths): bit-exact entry ids in the same order."""
+    print(f"case: cell lists vs reference tile lists, sizes={sizes}", flush=True)
+    model = S.free_scene(20000, seed=6, layout="mixture")
+    cams = _cameras(sizes, seed=19)
+    r, params = _batched(model, cams, _bgs(len(cams)))
+    r.forward(params)
+    torch.cuda.synchronize()
+    ranges, keys, vals = r.cell_lists()
+    ranges, keys, vals = ranges.numpy(), keys.numpy().view(np.uint32), vals.numpy().view(np.uint32)
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0), "cell ids must be sorted"
+    o = r._offsets[5]
+    splats = r.state[o:o + 48 * r.B * r.P].view(torch.float32).view(r.B * r.P, 12).cpu().numpy()
+    ids = vals & 0x0FFFFFFF
+    tags = vals >> 28
+    all_radii = r.radii().cpu().numpy()  # splat records of culled entries are not written
+    checked = 0
+    for v in range(r.B):
+        d = r.views[v]
+        H, W = r.sizes[v]
+        sl = slice(v * r.P, (v + 1) * r.P)
+        xy = splats[sl, 0:2].copy()
+        radii = all_radii[v]
+        assert np.array_equal(radii[radii > 0], splats[sl, 10].copy().view(np.int32)[radii > 0])
+        depth = splats[sl, 9].copy()
+        _, ovals, oranges = OR.bin_gaussians(xy, radii, depth, W, H)
+        gx, gy = d.grid_x, d.grid_y
+        cgx = (gx + 1) // 2
+        for t in range(gx * gy):
+            tx, ty = t % gx, t // gx
+            cell = d.cell_base + (ty // 2) * cgx + tx // 2
+            bit = 1 << ((ty & 1) * 2 + (tx & 1))
+            s0, s1 = ranges[cell]
+            mine = ids[s0:s1][(tags[s0:s1] & bit) != 0].astype(np.int64) - v * r.P
+            ref = ovals[oranges[t, 0]:oranges[t, 1]]
+            assert np.array_equal(mine, ref), f"view {v} tile {t}: {mine[:8]} vs {ref[:8]}"
+            checked += ref.size
+    assert checked > 0
