@@ -123,8 +123,18 @@ struct RenderArgs {
   float* gacc;           // (P, DMGS_GACC_STRIDE)
 };
 // Work-group shape / LDS batch depth of the blend kernels (results are the
-// same for every variant; the autotuner picks the fastest).
-enum RenderVariant { kRender16x256 = 0, kRender16x128 = 1, kRender8x64 = 2, kRender8x128 = 3, kRenderVariants = 4 };
+// same for every variant; the autotuner picks the fastest).  The strip
+// variants run the backward with one wave per tile (four pixels per lane)
+// and the forward as kRender16x256 / kRender16x128.
+enum RenderVariant {
+  kRender16x256 = 0,
+  kRender16x128 = 1,
+  kRender8x64 = 2,
+  kRender8x128 = 3,
+  kRenderStrips64 = 4,
+  kRenderStrips128 = 5,
+  kRenderVariants = 6
+};
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
 void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
 

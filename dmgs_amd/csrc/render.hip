@@ -15,13 +15,20 @@
 //
 // Backward: the list is walked from the back, starting at the largest
 // n_contrib of the workgroup (entries behind every pixel's last contributor
-// cannot contribute), transmittance is recomputed by division instead of
-// being stored, and each wave reduces its 9 per-splat partial gradients with
-// DPP row sums into a per-batch LDS accumulator; the workgroup flushes that
-// accumulator with one global atomic per (splat, value) per batch.
+// cannot contribute), transmittance is recomputed (times 1 / (1 - alpha), one
+// reciprocal) instead of being stored, and each wave reduces its 9 per-splat
+// partial gradients with DPP row sums into a per-batch LDS accumulator; the
+// workgroup flushes that accumulator with one global atomic per (splat,
+// value) per batch.
 //
 // Several views in one launch: a global tile id indexes the ranges; the view
 // table gives each tile's image size, background and packed output offset.
+//
+// Footprint skip: staging also stores the bounding box of the splat's
+// alpha >= 1/255 region, the ellipse  d^T Q d <= 2 ln(255 o)  (Q the conic),
+// widened by one pixel.  A wave whose 64 pixels all lie outside the box
+// skips the splat: no pixel of it could pass the alpha test, so skipping
+// changes nothing (the splat is still counted in the list positions).
 #include "dmgs_common.h"
 #include "dmgs_kernels.h"
 
@@ -32,7 +39,35 @@ struct SplatLDS {
   float2 xy[BATCH];
   float4 con_o[BATCH];   // conic a, b, c, opacity
   float4 rgb_id[BATCH];  // r, g, b, id (bits)
+  float4 box[BATCH];     // alpha footprint: x_lo, x_hi, y_lo, y_hi (pixels)
 };
+
+// Bounding box of { d : o exp(-d^T Q d / 2) >= 1/255 } around xy, plus one
+// pixel; empty when o < 1/255 (the splat can never pass the alpha test).
+__device__ __forceinline__ float4 alpha_box(float x, float y, float a, float b, float c, float o) {
+  const float tau = 2.0f * __logf(255.0f * o);
+  if (!(tau > 0.0f)) return make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+  const float det = a * c - b * b;
+  const float hx = sqrtf(tau * c / det) + 1.0f, hy = sqrtf(tau * a / det) + 1.0f;
+  return make_float4(x - hx, x + hx, y - hy, y + hy);
+}
+
+// Pixel-centre rectangle of this lane's wave (lanes are row-major in the wave).
+struct WaveRect {
+  float x0, x1, y0, y1;
+};
+__device__ __forceinline__ WaveRect wave_rect(int px, int py) {
+  WaveRect w;
+  w.x0 = (float)__builtin_amdgcn_readfirstlane(px);
+  w.y0 = (float)__builtin_amdgcn_readfirstlane(py);
+  w.x1 = (float)__builtin_amdgcn_readlane(px, 63);
+  w.y1 = (float)__builtin_amdgcn_readlane(py, 63);
+  return w;
+}
+__device__ __forceinline__ bool outside(const float4& b, const WaveRect& w) {
+  // one 16-byte LDS read and no short-circuit branches
+  return (b.x > w.x1) | (b.y < w.x0) | (b.z > w.y1) | (b.w < w.y0);
+}
 
 template <int BATCH>
 __device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const float4* __restrict__ splats,
@@ -41,6 +76,7 @@ __device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const 
     L.xy[slot] = make_float2(0.f, 0.f);
     L.con_o[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     L.rgb_id[slot] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
+    L.box[slot] = make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
     return;
   }
   const float4* sp = splats + (size_t)DMGS_SPLAT_F4 * id;
@@ -50,6 +86,7 @@ __device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const 
   L.xy[slot] = make_float2(s0.x, s0.y);
   L.con_o[slot] = make_float4(s0.z, s0.w, s1.x, s1.y);
   L.rgb_id[slot] = make_float4(s1.z, s1.w, s2.x, __uint_as_float(id));
+  L.box[slot] = alpha_box(s0.x, s0.y, s0.z, s0.w, s1.x, s1.y);
 }
 
 // Everything a workgroup needs to know about its pixels.
@@ -165,7 +202,8 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_kernel(RenderArgs a) {
   bool done = !c.inside;
   float T = 1.0f;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f;
-  uint32_t contributor = 0, last_contributor = 0;
+  uint32_t last_contributor = 0;
+  const WaveRect wr = wave_rect(c.px, c.py);
 
   for (int r = 0; r < rounds; ++r) {
     if (__syncthreads_count(done) == NT) break;
@@ -176,7 +214,8 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_kernel(RenderArgs a) {
     __syncthreads();
     const int n = min(BATCH, todo_total - r * BATCH);
     for (int j = 0; !done && j < n; ++j) {
-      ++contributor;
+      if (outside(L.box[j], wr)) continue;
+      const uint32_t contributor = (uint32_t)(r * BATCH + j + 1);
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
@@ -226,11 +265,13 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a
   float T = 1.0f;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f;
   uint32_t last_contributor = 0;
+  const WaveRect wr = wave_rect(c.px, c.py);
 
   for (int base = 0; base < len; base += BATCH) {
     if (__syncthreads_count(done) == NT) break;
     const int n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, base, len);
     for (int j = 0; !done && j < n; ++j) {
+      if (outside(L.box[j], wr)) continue;
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
@@ -303,11 +344,11 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
   const int rounds = (todo_total + BATCH - 1) / BATCH;
 
   float T = T_final;
-  uint32_t contributor = (uint32_t)todo_total;
   float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
   float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
 
   const unsigned row_lane = lane & 15;  // lane within its DPP row of 16
+  const WaveRect wr = wave_rect(c.px, c.py);
   for (int r = 0; r < rounds; ++r) {
     __syncthreads();
     int n;
@@ -328,53 +369,54 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
       n = min(BATCH, todo_total - r * BATCH);
     }
     for (int j = 0; j < n; ++j) {
+      if (outside(L.box[j], wr)) continue;  // no pixel of this wave can pass the alpha test
+      uint32_t contributor;
       if constexpr (CELLS) {
         contributor = s_pos[j];
       } else {
-        --contributor;
+        contributor = (uint32_t)(todo_total - 1 - (r * BATCH + j));
       }
-      float v[DMGS_GACC_USED];
-#pragma unroll
-      for (int k = 0; k < DMGS_GACC_USED; ++k) v[k] = 0.f;
-      bool contrib = false;
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
-      if (contributor < last_contributor) {
-        const float dx = xy.x - pxf, dy = xy.y - pyf;
-        const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
-        const float G = __expf(power);
-        const float alpha = fminf(0.99f, co.w * G);
-        if (power <= 0.0f && alpha >= 1.0f / 255.0f) {
-          contrib = true;
-          const float4 col = L.rgb_id[j];
-          T = T / (1.f - alpha);
-          const float dchannel_dcolor = alpha * T;
-          acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
-          acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
-          acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
-          lc0 = col.x;
-          lc1 = col.y;
-          lc2 = col.z;
-          float dL_dalpha = (col.x - acc0) * g0 + (col.y - acc1) * g1 + (col.z - acc2) * g2;
-          v[6] = dchannel_dcolor * g0;
-          v[7] = dchannel_dcolor * g1;
-          v[8] = dchannel_dcolor * g2;
-          dL_dalpha *= T;
-          last_alpha = alpha;
-          dL_dalpha += (-T_final / (1.f - alpha)) * bg_dot;
-          const float dL_dG = co.w * dL_dalpha;
-          const float gdx = G * dx, gdy = G * dy;
-          const float dG_ddelx = -gdx * co.x - gdy * co.y;
-          const float dG_ddely = -gdy * co.z - gdx * co.y;
-          v[0] = dL_dG * dG_ddelx * ddelx_dx;
-          v[1] = dL_dG * dG_ddely * ddely_dy;
-          v[2] = -0.5f * gdx * dx * dL_dG;
-          v[3] = -0.5f * gdx * dy * dL_dG;
-          v[4] = -0.5f * gdy * dy * dL_dG;
-          v[5] = G * dL_dalpha;
-        }
-      }
+      const float dx = xy.x - pxf, dy = xy.y - pyf;
+      const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+      const float G = __expf(power);
+      const float alpha = fminf(0.99f, co.w * G);
+      const bool contrib = contributor < last_contributor && power <= 0.0f && alpha >= 1.0f / 255.0f;
       if (__ballot(contrib) == 0) continue;  // wave-uniform
+      float v[DMGS_GACC_USED];
+      if (contrib) {
+        const float4 col = L.rgb_id[j];
+        const float inv_one_m = __builtin_amdgcn_rcpf(1.f - alpha);  // T before this splat = T / (1 - alpha)
+        T = T * inv_one_m;
+        const float dchannel_dcolor = alpha * T;
+        acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
+        acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
+        acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
+        lc0 = col.x;
+        lc1 = col.y;
+        lc2 = col.z;
+        float dL_dalpha = (col.x - acc0) * g0 + (col.y - acc1) * g1 + (col.z - acc2) * g2;
+        v[6] = dchannel_dcolor * g0;
+        v[7] = dchannel_dcolor * g1;
+        v[8] = dchannel_dcolor * g2;
+        dL_dalpha *= T;
+        last_alpha = alpha;
+        dL_dalpha += -T_final * inv_one_m * bg_dot;
+        const float dL_dG = co.w * dL_dalpha;
+        const float gdx = G * dx, gdy = G * dy;
+        const float dG_ddelx = -gdx * co.x - gdy * co.y;
+        const float dG_ddely = -gdy * co.z - gdx * co.y;
+        v[0] = dL_dG * dG_ddelx * ddelx_dx;
+        v[1] = dL_dG * dG_ddely * ddely_dy;
+        v[2] = -0.5f * gdx * dx * dL_dG;
+        v[3] = -0.5f * gdx * dy * dL_dG;
+        v[4] = -0.5f * gdy * dy * dL_dG;
+        v[5] = G * dL_dalpha;
+      } else {
+#pragma unroll
+        for (int k = 0; k < DMGS_GACC_USED; ++k) v[k] = 0.f;
+      }
       // sum over each 16-lane row, then lane k of every row adds value k into
       // the batch accumulator in LDS (one ds_add with 4 x 9 active lanes)
       float mine = 0.f;
@@ -393,6 +435,150 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
       if (val != 0.f) {
         const uint32_t gid = __float_as_uint(L.rgb_id[slot].w);
         unsafeAtomicAdd(a.gacc + (size_t)gid * DMGS_GACC_STRIDE + k, val);
+      }
+    }
+  }
+}
+
+// Backward with one wave per 16x16 tile: lane (lx, ly) owns the pixels
+// (lx, ly + 4k), k = 0..3, i.e. one pixel in each of four 16x4 strips.  The
+// footprint test runs per strip, each pixel is evaluated exactly as in
+// render_bwd_kernel, and a lane first sums its four pixels' partial
+// gradients, so each (tile, splat) pays for one wave reduction instead of
+// four.
+template <int BATCH, bool CELLS>
+__global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
+  constexpr int NT = 64;
+  __shared__ SplatLDS<BATCH> L;
+  __shared__ float s_acc[BATCH][DMGS_GACC_USED];
+  __shared__ uint32_t s_pos[CELLS ? BATCH : 1];
+  __shared__ uint32_t s_wc[1];
+  const TileCtx c = tile_ctx<16>(a);  // pixel k = 0 of this lane: (lx, ly), ly in 0..3
+  const unsigned lane = threadIdx.x & 63;
+  const float pxf = (float)c.px;
+  const float ddelx_dx = 0.5f * (float)c.W, ddely_dy = 0.5f * (float)c.H;
+
+  float T[4], T_final[4], g0[4], g1[4], g2[4], bg_dot[4];
+  float acc0[4], acc1[4], acc2[4], last_alpha[4], lc0[4], lc1[4], lc2[4];
+  uint32_t last_contributor[4];
+  WaveRect strip[4];
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int py = c.py + 4 * k;
+    const bool inside = c.px < c.W && py < c.H;
+    T_final[k] = 0.f;
+    g0[k] = g1[k] = g2[k] = 0.f;
+    last_contributor[k] = 0;
+    if (inside) {
+      const size_t pix = c.pix + (size_t)(4 * k) * c.W;
+      const size_t pid = pix - c.img / 3;
+      const float* dp = a.dL_dpix + c.img;
+      T_final[k] = a.final_T[pix];
+      last_contributor[k] = a.n_contrib[pix];
+      g0[k] = dp[pid];
+      g1[k] = dp[c.plane + pid];
+      g2[k] = dp[2 * c.plane + pid];
+    }
+    bg_dot[k] = c.bg0 * g0[k] + c.bg1 * g1[k] + c.bg2 * g2[k];
+    T[k] = T_final[k];
+    acc0[k] = acc1[k] = acc2[k] = 0.f;
+    last_alpha[k] = lc0[k] = lc1[k] = lc2[k] = 0.f;
+    strip[k] = wave_rect(c.px, py);
+    m = max(m, last_contributor[k]);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+  const int todo_total = min((int)(c.range.y - c.range.x), (int)m);
+  const int rounds = (todo_total + BATCH - 1) / BATCH;
+
+  const unsigned row_lane = lane & 15;
+  for (int r = 0; r < rounds; ++r) {
+    __syncthreads();
+    int n;
+    if constexpr (CELLS) {
+      for (int s = threadIdx.x; s < BATCH; s += NT) {
+#pragma unroll
+        for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
+      }
+      n = stage_cell_batch<NT, BATCH, true>(L, s_pos, s_wc, a, c, todo_total - r * BATCH, 0);
+    } else {
+      for (int s = threadIdx.x; s < BATCH; s += NT) {
+        const int pos = r * BATCH + s;
+        if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P);
+#pragma unroll
+        for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
+      }
+      __syncthreads();
+      n = min(BATCH, todo_total - r * BATCH);
+    }
+    for (int j = 0; j < n; ++j) {
+      const float4 box = L.box[j];
+      uint32_t contributor;
+      if constexpr (CELLS) {
+        contributor = s_pos[j];
+      } else {
+        contributor = (uint32_t)(todo_total - 1 - (r * BATCH + j));
+      }
+      const float2 xy = L.xy[j];
+      const float4 co = L.con_o[j];
+      float v[DMGS_GACC_USED];
+#pragma unroll
+      for (int q = 0; q < DMGS_GACC_USED; ++q) v[q] = 0.f;
+      bool any = false;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (outside(box, strip[k])) continue;  // wave-uniform
+        const float dx = xy.x - pxf, dy = xy.y - (float)(c.py + 4 * k);
+        const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+        const float G = __expf(power);
+        const float alpha = fminf(0.99f, co.w * G);
+        if (contributor < last_contributor[k] && power <= 0.0f && alpha >= 1.0f / 255.0f) {
+          any = true;
+          const float4 col = L.rgb_id[j];
+          const float inv_one_m = __builtin_amdgcn_rcpf(1.f - alpha);
+          T[k] = T[k] * inv_one_m;
+          const float dchannel_dcolor = alpha * T[k];
+          acc0[k] = last_alpha[k] * lc0[k] + (1.f - last_alpha[k]) * acc0[k];
+          acc1[k] = last_alpha[k] * lc1[k] + (1.f - last_alpha[k]) * acc1[k];
+          acc2[k] = last_alpha[k] * lc2[k] + (1.f - last_alpha[k]) * acc2[k];
+          lc0[k] = col.x;
+          lc1[k] = col.y;
+          lc2[k] = col.z;
+          float dL_dalpha = (col.x - acc0[k]) * g0[k] + (col.y - acc1[k]) * g1[k] + (col.z - acc2[k]) * g2[k];
+          v[6] += dchannel_dcolor * g0[k];
+          v[7] += dchannel_dcolor * g1[k];
+          v[8] += dchannel_dcolor * g2[k];
+          dL_dalpha *= T[k];
+          last_alpha[k] = alpha;
+          dL_dalpha += -T_final[k] * inv_one_m * bg_dot[k];
+          const float dL_dG = co.w * dL_dalpha;
+          const float gdx = G * dx, gdy = G * dy;
+          const float dG_ddelx = -gdx * co.x - gdy * co.y;
+          const float dG_ddely = -gdy * co.z - gdx * co.y;
+          v[0] += dL_dG * dG_ddelx * ddelx_dx;
+          v[1] += dL_dG * dG_ddely * ddely_dy;
+          v[2] += -0.5f * gdx * dx * dL_dG;
+          v[3] += -0.5f * gdx * dy * dL_dG;
+          v[4] += -0.5f * gdy * dy * dL_dG;
+          v[5] += G * dL_dalpha;
+        }
+      }
+      if (__ballot(any) == 0) continue;
+      float mine = 0.f;
+#pragma unroll
+      for (int q = 0; q < DMGS_GACC_USED; ++q) {
+        const float sum = row16_sum(v[q]);
+        mine = (row_lane == (unsigned)q) ? sum : mine;
+      }
+      if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[j][row_lane], mine);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n * DMGS_GACC_USED; i += NT) {
+      const int slot = i / DMGS_GACC_USED, q = i - slot * DMGS_GACC_USED;
+      const float val = s_acc[slot][q];
+      if (val != 0.f) {
+        const uint32_t gid = __float_as_uint(L.rgb_id[slot].w);
+        unsafeAtomicAdd(a.gacc + (size_t)gid * DMGS_GACC_STRIDE + q, val);
       }
     }
   }
@@ -417,10 +603,19 @@ static void launch_bwd_t(const RenderArgs& a, hipStream_t s) {
                        a);
 }
 
+template <int BATCH>
+static void launch_bwd_strips_t(const RenderArgs& a, hipStream_t s) {
+  if (a.views)
+    hipLaunchKernelGGL((render_bwd_strips_kernel<BATCH, true>), dim3(a.num_tiles), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((render_bwd_strips_kernel<BATCH, false>), dim3(a.num_tiles), dim3(64), 0, s, a);
+}
+
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {
   if (a.num_tiles <= 0) return;
   switch (variant) {
-    case kRender16x128: launch_fwd_t<16, 128>(a, s); break;
+    case kRender16x128:
+    case kRenderStrips128: launch_fwd_t<16, 128>(a, s); break;
     case kRender8x64: launch_fwd_t<8, 64>(a, s); break;
     case kRender8x128: launch_fwd_t<8, 128>(a, s); break;
     default: launch_fwd_t<16, 256>(a, s); break;
@@ -430,6 +625,8 @@ void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {
 void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant) {
   if (a.num_tiles <= 0) return;
   switch (variant) {
+    case kRenderStrips64: launch_bwd_strips_t<64>(a, s); break;
+    case kRenderStrips128: launch_bwd_strips_t<128>(a, s); break;
     case kRender16x128: launch_bwd_t<16, 128>(a, s); break;
     case kRender8x64: launch_bwd_t<8, 64>(a, s); break;
     case kRender8x128: launch_bwd_t<8, 128>(a, s); break;
