@@ -183,8 +183,8 @@ class BatchedRenderer:
             K_cap = max(1024, int(math.ceil(K * self.headroom)))
         self.K_cap = int(K_cap)
         self.lists = torch.empty(self.lib.dmgs_batched_lists_size(self.K_cap), dtype=torch.uint8, device=self.device)
-        if backward and self.bwd_ws is None:
-            self.bwd_ws = torch.empty(self.lib.dmgs_batched_backward_size(self.B, self.P), dtype=torch.uint8,
+        if backward and self.bwd_ws is None:  # zero once; each full backward leaves it zeroed
+            self.bwd_ws = torch.zeros(self.lib.dmgs_batched_backward_size(self.B, self.P), dtype=torch.uint8,
                                       device=self.device)
         return self.K_cap
 

@@ -86,6 +86,8 @@ template <typename T>
 T* at(void* base, size_t off) { return reinterpret_cast<T*>(static_cast<char*>(base) + off); }
 template <typename T>
 const T* at(const void* base, size_t off) { return reinterpret_cast<const T*>(static_cast<const char*>(base) + off); }
+// Sub-buffers sit at 256-byte offsets and kernels use 16-byte vector accesses.
+bool misaligned(const void* p) { return p && (reinterpret_cast<uintptr_t>(p) & 255u) != 0; }
 
 int tile_bits(int num_tiles) {
   int b = 0;
@@ -187,6 +189,7 @@ int dmgs_forward_geometry(const dmgs_settings* s, int64_t P, const float* means3
     return DMGS_OK;
   }
   if (!radii || !geom_ws) return fail(DMGS_EINVAL, "radii and geom_ws are required");
+  if (misaligned(geom_ws)) return fail(DMGS_EINVAL, "geom_ws must be 256-byte aligned");
   GeomLayout L((size_t)P);
   dmgs::PreprocessArgs a{};
   a.P = (int)P;
@@ -236,6 +239,8 @@ int dmgs_forward_binning(const dmgs_settings* s, int64_t P, int64_t K, const voi
     return fail(DMGS_EINVAL, "invalid P / K");
   if (!image_ws) return fail(DMGS_EINVAL, "image_ws is required");
   if (K > 0 && (!geom_ws || !binning_ws)) return fail(DMGS_EINVAL, "geom_ws and binning_ws are required");
+  if (misaligned(geom_ws) || misaligned(binning_ws) || misaligned(image_ws))
+    return fail(DMGS_EINVAL, "workspaces must be 256-byte aligned");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   float fx, fy;
   int gx, gy;
@@ -279,6 +284,8 @@ int dmgs_render_forward(const dmgs_settings* s, int64_t P, int64_t K, const void
     return fail(DMGS_EINVAL, "invalid P / K");
   if (!image_ws || !out_color) return fail(DMGS_EINVAL, "image_ws and out_color are required");
   if (K > 0 && (!geom_ws || !binning_ws)) return fail(DMGS_EINVAL, "geom_ws and binning_ws are required");
+  if (misaligned(geom_ws) || misaligned(binning_ws) || misaligned(image_ws))
+    return fail(DMGS_EINVAL, "workspaces must be 256-byte aligned");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   float fx, fy;
   int gx, gy;
@@ -334,6 +341,8 @@ int dmgs_render_backward(const dmgs_settings* s, int64_t P, int64_t K, const voi
     return fail(DMGS_EINVAL, "invalid P / K");
   if (P == 0) return DMGS_OK;
   if (!geom_ws || !image_ws || !bwd_ws || !dL_dout) return fail(DMGS_EINVAL, "missing buffers");
+  if (misaligned(geom_ws) || misaligned(binning_ws) || misaligned(image_ws) || misaligned(bwd_ws))
+    return fail(DMGS_EINVAL, "workspaces must be 256-byte aligned");
   if (K > 0 && !binning_ws) return fail(DMGS_EINVAL, "binning_ws is required");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   float fx, fy;

@@ -114,6 +114,9 @@ int check_batch(const dmgs_batch* b) {
   if (b->total_tiles <= 0 || b->total_pixels <= 0) return bfail(DMGS_EINVAL, "invalid view totals");
   if (b->sh_degree < 0 || b->sh_degree > 3) return bfail(DMGS_EINVAL, "sh_degree must be in [0, 3]");
   if (!b->views || !b->state) return bfail(DMGS_EINVAL, "views and state are required");
+  const auto misaligned = [](const void* p) { return p && (reinterpret_cast<uintptr_t>(p) & 255u) != 0; };
+  if (misaligned(b->state) || misaligned(b->lists) || misaligned(b->bwd_ws))
+    return bfail(DMGS_EINVAL, "workspaces must be 256-byte aligned");
   return DMGS_OK;
 }
 
@@ -303,8 +306,7 @@ int dmgs_batched_backward(const dmgs_batch* b, const float* params, const float*
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const Ctx c(b);
   dmgs::MVArgs a = mv_args(b, c, params);
-  if (stages & DMGS_STAGE_BLEND) {
-    dmgs::launch_mv_clear_gacc(a, st);
+  if (stages & DMGS_STAGE_BLEND) {  // accumulates into bwd_ws, which holds zeros on entry
     dmgs::RenderArgs r = render_args(b, c);
     r.dL_dpix = dL_dimages;
     dmgs::launch_render_bwd(r, st, b->variant);

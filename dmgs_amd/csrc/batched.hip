@@ -86,19 +86,6 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
   }
 }
 
-// Zero the blend-backward accumulator of every visible (view, Gaussian)
-// entry; the blend backward only adds into entries that appear in the lists.
-__global__ __launch_bounds__(256) void mv_clear_gacc_kernel(const int32_t* __restrict__ radii,
-                                                            float* __restrict__ gacc, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || radii[i] <= 0) return;
-  float4* gz = reinterpret_cast<float4*>(gacc + i * DMGS_GACC_STRIDE);
-  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-  gz[0] = z;
-  gz[1] = z;
-  gz[2] = z;
-}
-
 __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.P) return;
@@ -117,10 +104,15 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
     any = true;
     const ViewDesc& vd = a.views[v];
     const ViewParams vp = view_params(vd);
-    const float* gr = a.gacc + i * DMGS_GACC_STRIDE;
+    float* gr = a.gacc + i * DMGS_GACC_STRIDE;
     const float4 g03 = *reinterpret_cast<const float4*>(gr);
     const float4 g47 = *reinterpret_cast<const float4*>(gr + 4);
     const float g8 = gr[8];
+    // hand the row back zeroed: the next blend backward accumulates into zeros
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    reinterpret_cast<float4*>(gr)[0] = z;
+    reinterpret_cast<float4*>(gr)[1] = z;
+    reinterpret_cast<float4*>(gr)[2] = z;
     const float3 dm = ewa_backward(G.p, G.c3, vp, g03.x, g03.y, g03.z, g03.w, g47.x, dc);
     float b[16];
     float3 gm;
@@ -170,12 +162,6 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s) {
   if (a.P <= 0 || a.B <= 0) return;
   hipLaunchKernelGGL(mv_preprocess_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
-}
-
-void launch_mv_clear_gacc(const MVArgs& a, hipStream_t s) {
-  const size_t n = (size_t)a.B * a.P;
-  if (n == 0) return;
-  hipLaunchKernelGGL(mv_clear_gacc_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.radii, a.gacc, n);
 }
 
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s) {

@@ -93,11 +93,11 @@ struct MVArgs {
   uint32_t* tiles;           // (B*P)
   uint32_t* depth_keys;      // (B*P)
   uint32_t* order;           // (B*P)
-  float* gacc;               // (B*P, 16) blend-backward accumulator
+  float* gacc;               // (B*P, 16) blend-backward accumulator; zero between backward passes
   float* grads;              // (59, P)
 };
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s);
-void launch_mv_clear_gacc(const MVArgs& a, hipStream_t s);  // zero gacc rows of visible entries
+// Reads the accumulator rows of visible entries and leaves them zeroed.
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s);
 
 // ---- blend ----

@@ -479,30 +479,50 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   if (n) hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
 }
 
+// Four sorted keys per thread (one 16-byte load), each compared with its
+// predecessor: a key that differs from the previous one closes the previous
+// tile's range and opens its own.
 __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, uint2* __restrict__ ranges,
                                    uint32_t num_tiles, const uint32_t* __restrict__ K_dev) {
   if (K_dev) K = min(K, (size_t)*K_dev);
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= K) return;
-  const uint32_t t = keys[i];
-  if (t >= num_tiles) return;  // cannot happen for keys from the duplicate step
-  if (i == 0) {
-    ranges[t].x = 0;
+  const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (i0 >= K) return;
+  uint32_t k[4];
+  if (i0 + 4 <= K) {
+    const uint4 q = *reinterpret_cast<const uint4*>(keys + i0);
+    k[0] = q.x;
+    k[1] = q.y;
+    k[2] = q.z;
+    k[3] = q.w;
   } else {
-    const uint32_t prev = keys[i - 1];
-    if (t != prev && prev < num_tiles) {
-      ranges[prev].y = (uint32_t)i;
-      ranges[t].x = (uint32_t)i;
-    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) k[j] = (i0 + j < K) ? keys[i0 + j] : 0u;
   }
-  if (i == K - 1) ranges[t].y = (uint32_t)K;
+  uint32_t prev = (i0 == 0) ? 0u : keys[i0 - 1];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const size_t i = i0 + j;
+    if (i >= K) break;
+    const uint32_t t = k[j];
+    if (t < num_tiles) {  // always true for keys from the duplicate step
+      if (i == 0) {
+        ranges[t].x = 0;
+      } else if (t != prev && prev < num_tiles) {
+        ranges[prev].y = (uint32_t)i;
+        ranges[t].x = (uint32_t)i;
+      }
+      if (i == K - 1) ranges[t].y = (uint32_t)K;
+    }
+    prev = t;
+  }
 }
 
 void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s,
                         const uint32_t* K_dev) {
   launch_fill_u32((uint32_t*)ranges, (size_t)num_tiles * 2, 0u, s);
-  if (K) hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, tile_keys, K,
-                            ranges, (uint32_t)num_tiles, K_dev);
+  const size_t threads = (K + 3) / 4;
+  if (K) hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, tile_keys,
+                            K, ranges, (uint32_t)num_tiles, K_dev);
 }
 
 }  // namespace dmgs

@@ -5,7 +5,8 @@
  * contiguous float32 / int32 / uint32 memory that the caller owns, and every
  * call is enqueued on the caller's hipStream_t.  The library allocates
  * nothing: each workspace is sized by a *_workspace_size() query and passed
- * in as raw device memory (two-phase, like the reference's resize callbacks).
+ * in as raw device memory (two-phase, like the reference's resize callbacks),
+ * 256-byte aligned (hipMalloc and torch allocations are).
  *
  * Every entry point returns 0 on success, DMGS_EINVAL (-1) for an invalid
  * argument (nothing is launched) or DMGS_EHIP (-2) when a HIP call or launch
@@ -253,7 +254,10 @@ int dmgs_batched_tile_view(const dmgs_view* views, int32_t B, int32_t* tile_view
  *           per-tile ranges; per-pixel blend state.  Kept until backward.
  *   lists   the (Gaussian, tile) instance lists for up to `capacity`
  *           instances and their sort scratch.  Kept until backward.
- *   bwd_ws  per (view, Gaussian) gradient accumulator (backward only). */
+ *   bwd_ws  per (view, Gaussian) gradient accumulator (backward only).  It
+ *           must hold zeros before its first use (clear it once after
+ *           allocating); every backward that runs the GEOMETRY stage leaves
+ *           it zeroed again, so no clearing pass runs per step. */
 size_t dmgs_batched_state_size(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels);
 size_t dmgs_batched_lists_size(int64_t capacity);
 size_t dmgs_batched_backward_size(int32_t B, int64_t P);

@@ -113,6 +113,8 @@ def test_entry_points_validate_before_launch():
     assert lib.dmgs_batched_forward(_desc(num_views=2, num_gaussians=(1 << 27) + 1), fake, fake, A, None) == -1
     assert lib.dmgs_batched_forward(_desc(sh_degree=4), fake, fake, A, None) == -1
     assert "sh_degree" in N.last_error()
+    assert lib.dmgs_batched_forward(_desc(state=0x1004), fake, fake, A, None) == -1
+    assert "aligned" in N.last_error()
     assert lib.dmgs_batched_forward(_desc(total_cells=0), fake, fake, A, None) == -1
     assert lib.dmgs_batched_forward(_desc(total_cells=5), fake, fake, A, None) == -1  # more cells than tiles
     for bad in (0, 8, -1):
