@@ -68,8 +68,8 @@ int bits_for(uint32_t n) {
 // K-independent state of a batch: per-entry splats / keys / order / radii /
 // scan, the depth-sort scratch, per-tile ranges and per-pixel blend state.
 struct StateLayout {
-  size_t splats, keys_a, keys_b, order_a, order_b, tiles, radii, offsets, gscratch, ranges, final_T, n_contrib,
-      total;
+  size_t splats, keys_a, keys_b, order_a, order_b, rects, rects_sorted, radii, offsets, gscratch, ranges, final_T,
+      n_contrib, total;
   StateLayout(size_t N, size_t T, size_t X) {
     size_t o = 0;
     splats = o; o += align_up(N * DMGS_SPLAT_F4 * 16);
@@ -77,7 +77,8 @@ struct StateLayout {
     keys_b = o; o += align_up(N * 4);
     order_a = o; o += align_up(N * 4);
     order_b = o; o += align_up(N * 4);
-    tiles = o; o += align_up(N * 4);
+    rects = o; o += align_up(N * 8);
+    rects_sorted = o; o += align_up(N * 8);
     radii = o; o += align_up(N * 4);
     offsets = o; o += align_up(N * 4);
     gscratch = o; o += align_up(std::max(dmgs::radix_scratch_words(N), dmgs::scan_scratch_words(N)) * 4);
@@ -140,7 +141,7 @@ dmgs::MVArgs mv_args(const dmgs_batch* b, const Ctx& c, const float* params) {
   a.params = params;
   a.splats = at<float4>(b->state, c.S.splats);
   a.radii = at<int32_t>(b->state, c.S.radii);
-  a.tiles = at<uint32_t>(b->state, c.S.tiles);
+  a.rects = at<uint2>(b->state, c.S.rects);
   a.depth_keys = at<uint32_t>(b->state, c.S.keys_a);
   a.order = at<uint32_t>(b->state, c.S.order_a);
   a.gacc = static_cast<float*>(b->bwd_ws);
@@ -259,7 +260,7 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     uint32_t* gs = at<uint32_t>(b->state, c.S.gscratch);
     dmgs::radix_sort_pairs(a.depth_keys, a.order, at<uint32_t>(b->state, c.S.keys_b),
                            at<uint32_t>(b->state, c.S.order_b), N, 32, gs, st);
-    dmgs::launch_gather_u32(a.tiles, a.order, offs, (int)N, st);
+    dmgs::launch_gather_rects(a.rects, a.order, at<uint2>(b->state, c.S.rects_sorted), offs, N, st);
     dmgs::launch_inclusive_scan(offs, offs, N, gs, st);
     if ((rc = bcheck("batch preprocess / depth sort / scan", st))) return rc;
   }
@@ -270,6 +271,7 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     d.order = a.order;
     d.offsets = offs;
     d.splats = a.splats;
+    d.rects = at<uint2>(b->state, c.S.rects_sorted);
     d.views = a.views;
     d.per_view = (uint32_t)b->num_gaussians;
     d.tile_keys = at<uint32_t>(b->lists, c.L.tkeys_a);

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
     SplatGeom o;
     if (!project_splat(G.p, G.c3, vp, o)) {
       a.radii[i] = 0;
-      a.tiles[i] = 0;
+      a.rects[i] = make_uint2(0u, 0u);
       a.depth_keys[i] = 0xFFFFFFFFu;
       continue;
     }
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
     sp[1] = make_float4(o.conic.z, G.o, rgb.x, rgb.y);
     sp[2] = make_float4(rgb.z, o.depth, __int_as_float(o.radius), 0.0f);
     a.radii[i] = o.radius;
-    a.tiles[i] = (uint32_t)cell_count(o.rmin, o.rmax);  // instances are per binning cell
+    a.rects[i] = pack_rect(o.rmin, o.rmax);  // binning reads rectangles in depth order
     a.depth_keys[i] = __float_as_uint(o.depth);
   }
 }
@@ -157,6 +157,26 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
   put(DMGS_MV_OPACITY, dop * G.o * (1.0f - G.o));
 #pragma unroll
   for (int k = 0; k < 48; ++k) put(DMGS_MV_SH + k, dsh[k]);
+}
+
+__global__ __launch_bounds__(256) void gather_rects_kernel(const uint2* __restrict__ rects,
+                                                           const uint32_t* __restrict__ order,
+                                                           uint2* __restrict__ rects_sorted,
+                                                           uint32_t* __restrict__ counts, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint2 r = rects[order[i]];
+  rects_sorted[i] = r;
+  int2 rmin, rmax;
+  unpack_rect(r, rmin, rmax);
+  counts[i] = (rmax.x > rmin.x && rmax.y > rmin.y) ? (uint32_t)cell_count(rmin, rmax) : 0u;
+}
+
+void launch_gather_rects(const uint2* rects, const uint32_t* order, uint2* rects_sorted, uint32_t* counts, size_t n,
+                         hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(gather_rects_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rects, order,
+                     rects_sorted, counts, n);
 }
 
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s) {

@@ -97,6 +97,15 @@ __device__ __forceinline__ uint32_t cell_mask(int2 rmin, int2 rmax, int cx, int 
   return m;
 }
 
+// Tile rectangle [rmin, rmax) packed in 8 bytes: (x0 | x1 << 16, y0 | y1 << 16).
+__device__ __forceinline__ uint2 pack_rect(int2 rmin, int2 rmax) {
+  return make_uint2((uint32_t)rmin.x | ((uint32_t)rmax.x << 16), (uint32_t)rmin.y | ((uint32_t)rmax.y << 16));
+}
+__device__ __forceinline__ void unpack_rect(uint2 r, int2& rmin, int2& rmax) {
+  rmin = make_int2((int)(r.x & 0xFFFFu), (int)(r.y & 0xFFFFu));
+  rmax = make_int2((int)(r.x >> 16), (int)(r.y >> 16));
+}
+
 // --- wave64 reductions --------------------------------------------------
 // Sum over the 16 lanes of each DPP row; every lane of the row gets the sum.
 __device__ __forceinline__ float row16_sum(float v) {

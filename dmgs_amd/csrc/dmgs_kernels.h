@@ -59,7 +59,8 @@ struct DupArgs {
   int P;                        // depth-sorted entries
   const uint32_t* order;
   const uint32_t* offsets;      // inclusive prefix of tiles touched, in depth order
-  const float4* splats;         // radius in s2.z
+  const float4* splats;         // single view: radius in s2.z
+  const uint2* rects;           // several views: tile rectangles in depth order (pack_rect)
   int grid_x, grid_y;
   const ViewDesc* views;
   uint32_t per_view;
@@ -90,13 +91,16 @@ struct MVArgs {
   const float* params;       // (59, P)
   float4* splats;            // (B*P) records
   int32_t* radii;            // (B*P)
-  uint32_t* tiles;           // (B*P)
+  uint2* rects;              // (B*P) tile rectangle, pack_rect; 0 when culled
   uint32_t* depth_keys;      // (B*P)
   uint32_t* order;           // (B*P)
   float* gacc;               // (B*P, 16) blend-backward accumulator; zero between backward passes
   float* grads;              // (59, P)
 };
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s);
+// rects_sorted[i] = rects[order[i]], counts[i] = its number of binning cells.
+void launch_gather_rects(const uint2* rects, const uint32_t* order, uint2* rects_sorted, uint32_t* counts, size_t n,
+                         hipStream_t s);
 // Reads the accumulator rows of visible entries and leaves them zeroed.
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s);
 

@@ -394,16 +394,17 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
       const uint32_t beg = (s == 0) ? 0u : min(a.offsets[s - 1], K);
       end = min(a.offsets[s], K);
       if (end > beg && g < P) {
-        int gy = a.grid_y;
         const ViewDesc* v = a.views ? &a.views[g / a.per_view] : nullptr;
-        if (v) {
-          gx = v->grid_x;
-          gy = v->grid_y;
-        }
-        const float4 s0 = a.splats[(size_t)DMGS_SPLAT_F4 * g];
-        const int r = __float_as_int(a.splats[(size_t)DMGS_SPLAT_F4 * g + 2].z);
         int2 rmin, rmax;
-        tile_rect(make_float2(s0.x, s0.y), r, gx, gy, rmin, rmax);
+        if (v) {  // depth-ordered rectangle from the gather step
+          const uint2 pr = a.rects[s];
+          unpack_rect(pr, rmin, rmax);
+          gx = v->grid_x;
+        } else {
+          const float4 s0 = a.splats[(size_t)DMGS_SPLAT_F4 * g];
+          const int r = __float_as_int(a.splats[(size_t)DMGS_SPLAT_F4 * g + 2].z);
+          tile_rect(make_float2(s0.x, s0.y), r, gx, a.grid_y, rmin, rmax);
+        }
         if (v) {  // emit per binning cell of 2x2 tiles
           rx = (uint32_t)rmin.x | ((uint32_t)rmax.x << 16);
           ry = (uint32_t)rmin.y | ((uint32_t)rmax.y << 16);
