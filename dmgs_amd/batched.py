@@ -32,7 +32,8 @@ CHANNELS = 59
 XYZ, SCALE, ROT, OPACITY, SH = 0, 3, 6, 10, 11
 STAGE_GEOMETRY, STAGE_BINNING, STAGE_BLEND, STAGES_ALL = 1, 2, 4, 7
 RENDER_VARIANTS = ("16x16 px, 256-deep LDS batch", "16x16 px, 128-deep", "8x8 px, 64-deep", "8x8 px, 128-deep",
-                   "backward one wave per tile, 64-deep", "backward one wave per tile, 128-deep")
+                   "one wave per tile, 64-deep", "one wave per tile, 128-deep",
+                   "16x16 px 256-deep forward, one-wave-per-tile 64-deep backward")
 
 
 def pack_params(xyz, log_scale, rotation, opacity_logit, features_dc, features_rest, out=None):

@@ -369,7 +369,7 @@ int dmgs_render_backward(const dmgs_settings* s, int64_t P, int64_t K, const voi
   r.n_contrib = const_cast<uint32_t*>(at<uint32_t>(image_ws, IL.n_contrib));
   r.dL_dpix = dL_dout;
   r.gacc = gacc;
-  dmgs::launch_render_bwd(r, st, dmgs::kRenderStrips64);
+  dmgs::launch_render_bwd(r, st, dmgs::kRenderMixed);
   return check_launch("render backward", st, s->debug);
 }
 

@@ -128,8 +128,8 @@ struct RenderArgs {
 };
 // Work-group shape / LDS batch depth of the blend kernels (results are the
 // same for every variant; the autotuner picks the fastest).  The strip
-// variants run the backward with one wave per tile (four pixels per lane)
-// and the forward as kRender16x256 / kRender16x128.
+// variants run one wave per tile (four pixels per lane) forward and
+// backward; kRenderMixed pairs the 16x256 forward with the strip backward.
 enum RenderVariant {
   kRender16x256 = 0,
   kRender16x128 = 1,
@@ -137,7 +137,8 @@ enum RenderVariant {
   kRender8x128 = 3,
   kRenderStrips64 = 4,
   kRenderStrips128 = 5,
-  kRenderVariants = 6
+  kRenderMixed = 6,
+  kRenderVariants = 7
 };
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
 void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);

@@ -52,6 +52,14 @@ __device__ __forceinline__ float4 alpha_box(float x, float y, float a, float b, 
   return make_float4(x - hx, x + hx, y - hy, y + hy);
 }
 
+// power = -1/2 (a dx^2 + c dy^2) - b dx dy with a fixed rounding sequence
+// (explicit products, no compiler contraction), so every kernel variant, and
+// the backward's recomputation, gets bit-identical values.
+__device__ __forceinline__ float splat_power(const float4& co, float dx, float dy) {
+  const float q = fmaf(__fmul_rn(co.z, dy), dy, __fmul_rn(__fmul_rn(co.x, dx), dx));
+  return fmaf(-0.5f, q, -__fmul_rn(__fmul_rn(co.y, dx), dy));
+}
+
 // Pixel-centre rectangle of this lane's wave (lanes are row-major in the wave).
 struct WaveRect {
   float x0, x1, y0, y1;
@@ -67,6 +75,31 @@ __device__ __forceinline__ WaveRect wave_rect(int px, int py) {
 __device__ __forceinline__ bool outside(const float4& b, const WaveRect& w) {
   // one 16-byte LDS read and no short-circuit branches
   return (b.x > w.x1) | (b.y < w.x0) | (b.z > w.y1) | (b.w < w.y0);
+}
+
+#define DMGS_DPP_ADD(v, ctrl) ((v) + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), (ctrl), 0xF, 0xF, true)))
+
+// Row sums of nine per-lane values over each 16-lane DPP row; row lane k < 9
+// returns the sum of value k.  Three butterfly steps (quad swaps, rotate by 4)
+// on all nine values leave every lane the sum over its half of the row; each
+// lane then keeps value (lane & 7), so one rotate-by-8 step completes values
+// 0..7 (the partner lane holds the other half of the same value), and value
+// 8 takes one step of its own.  37 VALU instead of 54 for nine full row sums.
+__device__ __forceinline__ float row_sums9(const float (&v)[DMGS_GACC_USED], unsigned row_lane) {
+  float h[DMGS_GACC_USED];
+#pragma unroll
+  for (int k = 0; k < DMGS_GACC_USED; ++k) {
+    float x = DMGS_DPP_ADD(v[k], 0xB1);  // quad_perm 1,0,3,2
+    x = DMGS_DPP_ADD(x, 0x4E);           // quad_perm 2,3,0,1
+    h[k] = DMGS_DPP_ADD(x, 0x124);       // row_ror 4
+  }
+  const unsigned sel_k = row_lane & 7u;
+  float sel = h[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) sel = (sel_k == (unsigned)k) ? h[k] : sel;
+  sel = DMGS_DPP_ADD(sel, 0x128);        // row_ror 8
+  const float s8 = DMGS_DPP_ADD(h[8], 0x128);
+  return row_lane == 8u ? s8 : sel;
 }
 
 template <int BATCH>
@@ -219,7 +252,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_kernel(RenderArgs a) {
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
-      const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+      const float power = splat_power(co, dx, dy);
       if (power > 0.0f) continue;
       const float alpha = fminf(0.99f, co.w * __expf(power));
       if (alpha < 1.0f / 255.0f) continue;
@@ -275,7 +308,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
-      const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+      const float power = splat_power(co, dx, dy);
       if (power > 0.0f) continue;
       const float alpha = fminf(0.99f, co.w * __expf(power));
       if (alpha < 1.0f / 255.0f) continue;
@@ -301,6 +334,95 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a
     out[pid] = C0 + T * c.bg0;
     out[c.plane + pid] = C1 + T * c.bg1;
     out[2 * c.plane + pid] = C2 + T * c.bg2;
+  }
+}
+
+// Forward with one wave per 16x16 tile (the layout of the strip backward):
+// lane (lx, ly) owns the pixels (lx, ly + 4k), one per 16x4 strip.  Each
+// staged splat is read from LDS once per wave for four pixels; a strip is
+// skipped when the splat's footprint misses it or all its pixels are done.
+// Per pixel the arithmetic and order are render_fwd_kernel's.
+template <int BATCH, bool CELLS>
+__global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
+  constexpr int NT = 64;
+  __shared__ SplatLDS<BATCH> L;
+  __shared__ uint32_t s_pos[CELLS ? BATCH : 1];
+  __shared__ uint32_t s_wc[1];
+  const TileCtx c = tile_ctx<16>(a);
+  const float pxf = (float)c.px;
+  const int len = (int)(c.range.y - c.range.x);
+  float T[4], C0[4], C1[4], C2[4];
+  bool done[4];
+  uint32_t last_contributor[4];
+  WaveRect strip[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    done[k] = !(c.px < c.W && c.py + 4 * k < c.H);
+    T[k] = 1.0f;
+    C0[k] = C1[k] = C2[k] = 0.f;
+    last_contributor[k] = 0;
+    strip[k] = wave_rect(c.px, c.py + 4 * k);
+  }
+  const int rounds = (len + BATCH - 1) / BATCH;
+  for (int r = 0; r < rounds; ++r) {
+    if (__syncthreads_count(done[0] & done[1] & done[2] & done[3]) == NT) break;
+    int n;
+    if constexpr (CELLS) {
+      n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, r * BATCH, len);
+    } else {
+      for (int s = threadIdx.x; s < BATCH; s += NT) {
+        const int pos = r * BATCH + s;
+        if (pos < len) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P);
+      }
+      __syncthreads();
+      n = min(BATCH, len - r * BATCH);
+    }
+    for (int j = 0; j < n; ++j) {
+      const float4 box = L.box[j];
+      const float2 xy = L.xy[j];
+      const float4 co = L.con_o[j];
+      uint32_t position;
+      if constexpr (CELLS) {
+        position = s_pos[j];
+      } else {
+        position = (uint32_t)(r * BATCH + j);
+      }
+      const float dx = xy.x - pxf;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (outside(box, strip[k]) || __ballot(!done[k]) == 0) continue;  // wave-uniform
+        if (done[k]) continue;
+        const float dy = xy.y - (float)(c.py + 4 * k);
+        const float power = splat_power(co, dx, dy);
+        if (power > 0.0f) continue;
+        const float alpha = fminf(0.99f, co.w * __expf(power));
+        if (alpha < 1.0f / 255.0f) continue;
+        const float test_T = T[k] * (1.0f - alpha);
+        if (test_T < 0.0001f) {
+          done[k] = true;
+          continue;
+        }
+        const float4 col = L.rgb_id[j];
+        const float w = alpha * T[k];
+        C0[k] += col.x * w;
+        C1[k] += col.y * w;
+        C2[k] += col.z * w;
+        T[k] = test_T;
+        last_contributor[k] = position + 1;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (!(c.px < c.W && c.py + 4 * k < c.H)) continue;
+    const size_t pix = c.pix + (size_t)(4 * k) * c.W;
+    const size_t pid = pix - c.img / 3;
+    a.final_T[pix] = T[k];
+    a.n_contrib[pix] = last_contributor[k];
+    float* out = a.out_color + c.img;
+    out[pid] = C0[k] + T[k] * c.bg0;
+    out[c.plane + pid] = C1[k] + T[k] * c.bg1;
+    out[2 * c.plane + pid] = C2[k] + T[k] * c.bg2;
   }
 }
 
@@ -379,7 +501,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
-      const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+      const float power = splat_power(co, dx, dy);
       const float G = __expf(power);
       const float alpha = fminf(0.99f, co.w * G);
       const bool contrib = contributor < last_contributor && power <= 0.0f && alpha >= 1.0f / 255.0f;
@@ -419,12 +541,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
       }
       // sum over each 16-lane row, then lane k of every row adds value k into
       // the batch accumulator in LDS (one ds_add with 4 x 9 active lanes)
-      float mine = 0.f;
-#pragma unroll
-      for (int k = 0; k < DMGS_GACC_USED; ++k) {
-        const float s = row16_sum(v[k]);
-        mine = (row_lane == (unsigned)k) ? s : mine;
-      }
+      const float mine = row_sums9(v, row_lane);
       if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[j][row_lane], mine);
     }
     __syncthreads();
@@ -529,7 +646,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       for (int k = 0; k < 4; ++k) {
         if (outside(box, strip[k])) continue;  // wave-uniform
         const float dx = xy.x - pxf, dy = xy.y - (float)(c.py + 4 * k);
-        const float power = -0.5f * (co.x * dx * dx + co.z * dy * dy) - co.y * dx * dy;
+        const float power = splat_power(co, dx, dy);
         const float G = __expf(power);
         const float alpha = fminf(0.99f, co.w * G);
         if (contributor < last_contributor[k] && power <= 0.0f && alpha >= 1.0f / 255.0f) {
@@ -564,12 +681,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
         }
       }
       if (__ballot(any) == 0) continue;
-      float mine = 0.f;
-#pragma unroll
-      for (int q = 0; q < DMGS_GACC_USED; ++q) {
-        const float sum = row16_sum(v[q]);
-        mine = (row_lane == (unsigned)q) ? sum : mine;
-      }
+      const float mine = row_sums9(v, row_lane);
       if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[j][row_lane], mine);
     }
     __syncthreads();
@@ -604,6 +716,14 @@ static void launch_bwd_t(const RenderArgs& a, hipStream_t s) {
 }
 
 template <int BATCH>
+static void launch_fwd_strips_t(const RenderArgs& a, hipStream_t s) {
+  if (a.views)
+    hipLaunchKernelGGL((render_fwd_strips_kernel<BATCH, true>), dim3(a.num_tiles), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((render_fwd_strips_kernel<BATCH, false>), dim3(a.num_tiles), dim3(64), 0, s, a);
+}
+
+template <int BATCH>
 static void launch_bwd_strips_t(const RenderArgs& a, hipStream_t s) {
   if (a.views)
     hipLaunchKernelGGL((render_bwd_strips_kernel<BATCH, true>), dim3(a.num_tiles), dim3(64), 0, s, a);
@@ -614,8 +734,9 @@ static void launch_bwd_strips_t(const RenderArgs& a, hipStream_t s) {
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {
   if (a.num_tiles <= 0) return;
   switch (variant) {
-    case kRender16x128:
-    case kRenderStrips128: launch_fwd_t<16, 128>(a, s); break;
+    case kRender16x128: launch_fwd_t<16, 128>(a, s); break;
+    case kRenderStrips64: launch_fwd_strips_t<64>(a, s); break;
+    case kRenderStrips128: launch_fwd_strips_t<128>(a, s); break;
     case kRender8x64: launch_fwd_t<8, 64>(a, s); break;
     case kRender8x128: launch_fwd_t<8, 128>(a, s); break;
     default: launch_fwd_t<16, 256>(a, s); break;
@@ -625,7 +746,8 @@ void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {
 void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant) {
   if (a.num_tiles <= 0) return;
   switch (variant) {
-    case kRenderStrips64: launch_bwd_strips_t<64>(a, s); break;
+    case kRenderStrips64:
+    case kRenderMixed: launch_bwd_strips_t<64>(a, s); break;
     case kRenderStrips128: launch_bwd_strips_t<128>(a, s); break;
     case kRender16x128: launch_bwd_t<16, 128>(a, s); break;
     case kRender8x64: launch_bwd_t<8, 64>(a, s); break;
