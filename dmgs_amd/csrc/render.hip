@@ -39,17 +39,57 @@ struct SplatLDS {
   float2 xy[BATCH];
   float4 con_o[BATCH];   // conic a, b, c, opacity
   float4 rgb_id[BATCH];  // r, g, b, id (bits)
-  float4 box[BATCH];     // alpha footprint: x_lo, x_hi, y_lo, y_hi (pixels)
+  uint32_t mask[BATCH];  // pixel rectangles of the work-group the splat can reach (reach_mask)
 };
 
-// Bounding box of { d : o exp(-d^T Q d / 2) >= 1/255 } around xy, plus one
-// pixel; empty when o < 1/255 (the splat can never pass the alpha test).
-__device__ __forceinline__ float4 alpha_box(float x, float y, float a, float b, float c, float o) {
-  const float tau = 2.0f * __logf(255.0f * o);
-  if (!(tau > 0.0f)) return make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
-  const float det = a * c - b * b;
-  const float hx = sqrtf(tau * c / det) + 1.0f, hy = sqrtf(tau * a / det) + 1.0f;
-  return make_float4(x - hx, x + hx, y - hy, y + hy);
+// Which pixel rectangles of the work-group a splat can reach.  A splat passes
+// the alpha test only inside the ellipse  d^T Q d <= tau,  tau = 2 ln(255 o)
+// (o exp(-d^T Q d / 2) >= 1/255, Q the conic).  Staging tests that ellipse
+// against each rectangle of pixel centres a wave (or a strip of the tile)
+// covers and keeps one bit per rectangle; a wave skips the splat when its
+// bit is clear: no pixel there could pass the alpha test, so results are
+// unchanged.  Margins (rectangle +0.01 px, tau x 1.001 + 0.001) keep the test
+// conservative against rounding.
+struct RowRects {  // n rectangles [x0, x1] x [y0 + k h, y0 + k h + h - 1]
+  float x0, x1, y0, h;
+  int n;
+};
+
+// min over the rectangle of q(p - centre) <= tau, q(d) = a dx^2 + 2 b dx dy + c dy^2
+// (convex: the minimum is the centre if it is inside, else on an edge, where
+// the one-dimensional minimiser is clamped to the edge).
+__device__ __forceinline__ bool ellipse_meets_rect(float cx, float cy, float a, float b, float c, float tau,
+                                                   float x0, float x1, float y0, float y1) {
+  if (cx >= x0 && cx <= x1 && cy >= y0 && cy <= y1) return true;
+  const float ia = 1.0f / a, ic = 1.0f / c;
+  float best = INFINITY;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const float dx = (e ? x1 : x0) - cx;
+    const float dy = fminf(fmaxf(-b * dx * ic, y0 - cy), y1 - cy);
+    best = fminf(best, a * dx * dx + 2.0f * b * dx * dy + c * dy * dy);
+  }
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const float dy = (e ? y1 : y0) - cy;
+    const float dx = fminf(fmaxf(-b * dy * ia, x0 - cx), x1 - cx);
+    best = fminf(best, a * dx * dx + 2.0f * b * dx * dy + c * dy * dy);
+  }
+  return best <= tau;
+}
+
+__device__ __forceinline__ uint32_t reach_mask(float cx, float cy, float a, float b, float c, float o,
+                                               const RowRects& rr) {
+  const float tau = 2.0f * logf(255.0f * o);
+  if (!(tau > 0.0f)) return 0u;  // o < 1/255: alpha never reaches the threshold
+  if (!(a > 0.0f && c > 0.0f)) return 0xFu;
+  const float t = tau * 1.001f + 0.001f, m = 0.01f;
+  uint32_t mask = 0;
+  for (int k = 0; k < rr.n; ++k) {
+    const float y0 = rr.y0 + (float)k * rr.h;
+    if (ellipse_meets_rect(cx, cy, a, b, c, t, rr.x0 - m, rr.x1 + m, y0 - m, y0 + rr.h - 1.0f + m)) mask |= 1u << k;
+  }
+  return mask;
 }
 
 // power = -1/2 (a dx^2 + c dy^2) - b dx dy with a fixed rounding sequence
@@ -60,24 +100,12 @@ __device__ __forceinline__ float splat_power(const float4& co, float dx, float d
   return fmaf(-0.5f, q, -__fmul_rn(__fmul_rn(co.y, dx), dy));
 }
 
-// Pixel-centre rectangle of this lane's wave (lanes are row-major in the wave).
-struct WaveRect {
-  float x0, x1, y0, y1;
-};
-__device__ __forceinline__ WaveRect wave_rect(int px, int py) {
-  WaveRect w;
-  w.x0 = (float)__builtin_amdgcn_readfirstlane(px);
-  w.y0 = (float)__builtin_amdgcn_readfirstlane(py);
-  w.x1 = (float)__builtin_amdgcn_readlane(px, 63);
-  w.y1 = (float)__builtin_amdgcn_readlane(py, 63);
-  return w;
-}
-__device__ __forceinline__ bool outside(const float4& b, const WaveRect& w) {
-  // one 16-byte LDS read and no short-circuit branches
-  return (b.x > w.x1) | (b.y < w.x0) | (b.z > w.y1) | (b.w < w.y0);
-}
-
 #define DMGS_DPP_ADD(v, ctrl) ((v) + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), (ctrl), 0xF, 0xF, true)))
+
+// Wave-uniform test of bit k of a staged mask (read once, scalar branch).
+__device__ __forceinline__ bool reaches(uint32_t mask_lds, int k) {
+  return (__builtin_amdgcn_readfirstlane(mask_lds) >> k) & 1u;
+}
 
 // Row sums of nine per-lane values over each 16-lane DPP row; row lane k < 9
 // returns the sum of value k.  Three butterfly steps (quad swaps, rotate by 4)
@@ -104,12 +132,12 @@ __device__ __forceinline__ float row_sums9(const float (&v)[DMGS_GACC_USED], uns
 
 template <int BATCH>
 __device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const float4* __restrict__ splats,
-                                            uint32_t id, uint32_t P) {
+                                            uint32_t id, uint32_t P, const RowRects& rr) {
   if (id >= P) {  // cannot happen for lists built by the binning step; a zero-opacity entry never contributes
     L.xy[slot] = make_float2(0.f, 0.f);
     L.con_o[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     L.rgb_id[slot] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
-    L.box[slot] = make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+    L.mask[slot] = 0u;
     return;
   }
   const float4* sp = splats + (size_t)DMGS_SPLAT_F4 * id;
@@ -119,7 +147,7 @@ __device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const 
   L.xy[slot] = make_float2(s0.x, s0.y);
   L.con_o[slot] = make_float4(s0.z, s0.w, s1.x, s1.y);
   L.rgb_id[slot] = make_float4(s1.z, s1.w, s2.x, __uint_as_float(id));
-  L.box[slot] = alpha_box(s0.x, s0.y, s0.z, s0.w, s1.x, s1.y);
+  L.mask[slot] = reach_mask(s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, rr);
 }
 
 // Everything a workgroup needs to know about its pixels.
@@ -134,6 +162,7 @@ struct TileCtx {
   float bg0, bg1, bg2;
   uint32_t bit;        // multi-view: this tile's bit in its cell's instance tags
   uint2 range;         // single view: the tile's list; multi-view: its cell's list
+  RowRects rows;       // the work-group's pixel rectangles: 16x4 strips (SUB 16) or the 8x8 block
 };
 
 template <int SUB>
@@ -168,8 +197,16 @@ __device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
   }
   const int tx = lt % gx, ty = lt / gx;
   const int lx = threadIdx.x % SUB, ly = threadIdx.x / SUB;
-  c.px = tx * DMGS_BLOCK_X + (q % Q) * SUB + lx;
-  c.py = ty * DMGS_BLOCK_Y + (q / Q) * SUB + ly;
+  const int ox = tx * DMGS_BLOCK_X + (q % Q) * SUB, oy = ty * DMGS_BLOCK_Y + (q / Q) * SUB;
+  c.px = ox + lx;
+  c.py = oy + ly;
+  // rectangle k: the rows one wave covers in a 256-lane work-group (= strip k
+  // of a one-wave-per-tile work-group), or the whole 8x8 block
+  c.rows.x0 = (float)ox;
+  c.rows.x1 = (float)(ox + SUB - 1);
+  c.rows.y0 = (float)oy;
+  c.rows.h = SUB == 16 ? 4.0f : 8.0f;
+  c.rows.n = SUB == 16 ? 4 : 1;
   c.inside = c.px < c.W && c.py < c.H;
   c.plane = (size_t)c.W * c.H;
   c.pix = (size_t)pbase + (size_t)c.py * c.W + c.px;
@@ -214,7 +251,7 @@ __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_
     }
     if (member) {
       const int slot = before + (int)__popcll(bal & lt);
-      stage_splat(L, slot, a.splats, val & DMGS_CELL_ID_MASK, a.P);
+      stage_splat(L, slot, a.splats, val & DMGS_CELL_ID_MASK, a.P, c.rows);
       s_pos[slot] = (uint32_t)q;
     }
     run += total;
@@ -236,18 +273,18 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_kernel(RenderArgs a) {
   float T = 1.0f;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f;
   uint32_t last_contributor = 0;
-  const WaveRect wr = wave_rect(c.px, c.py);
+  const int my_rect = SUB == 16 ? (int)(threadIdx.x >> 6) : 0;  // this wave's rectangle in the reach masks
 
   for (int r = 0; r < rounds; ++r) {
     if (__syncthreads_count(done) == NT) break;
     for (int s = threadIdx.x; s < BATCH; s += NT) {
       const int pos = r * BATCH + s;
-      if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P);
+      if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
     }
     __syncthreads();
     const int n = min(BATCH, todo_total - r * BATCH);
     for (int j = 0; !done && j < n; ++j) {
-      if (outside(L.box[j], wr)) continue;
+      if (!reaches(L.mask[j], my_rect)) continue;
       const uint32_t contributor = (uint32_t)(r * BATCH + j + 1);
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
@@ -298,13 +335,13 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a
   float T = 1.0f;
   float C0 = 0.f, C1 = 0.f, C2 = 0.f;
   uint32_t last_contributor = 0;
-  const WaveRect wr = wave_rect(c.px, c.py);
+  const int my_rect = SUB == 16 ? (int)(threadIdx.x >> 6) : 0;  // this wave's rectangle in the reach masks
 
   for (int base = 0; base < len; base += BATCH) {
     if (__syncthreads_count(done) == NT) break;
     const int n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, base, len);
     for (int j = 0; !done && j < n; ++j) {
-      if (outside(L.box[j], wr)) continue;
+      if (!reaches(L.mask[j], my_rect)) continue;
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
@@ -354,14 +391,12 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
   float T[4], C0[4], C1[4], C2[4];
   bool done[4];
   uint32_t last_contributor[4];
-  WaveRect strip[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     done[k] = !(c.px < c.W && c.py + 4 * k < c.H);
     T[k] = 1.0f;
     C0[k] = C1[k] = C2[k] = 0.f;
     last_contributor[k] = 0;
-    strip[k] = wave_rect(c.px, c.py + 4 * k);
   }
   const int rounds = (len + BATCH - 1) / BATCH;
   for (int r = 0; r < rounds; ++r) {
@@ -372,13 +407,14 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = r * BATCH + s;
-        if (pos < len) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P);
+        if (pos < len) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
       }
       __syncthreads();
       n = min(BATCH, len - r * BATCH);
     }
     for (int j = 0; j < n; ++j) {
-      const float4 box = L.box[j];
+      const uint32_t reach = __builtin_amdgcn_readfirstlane(L.mask[j]);
+      if (reach == 0u) continue;
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
       uint32_t position;
@@ -390,7 +426,7 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
       const float dx = xy.x - pxf;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        if (outside(box, strip[k]) || __ballot(!done[k]) == 0) continue;  // wave-uniform
+        if (!((reach >> k) & 1u) || __ballot(!done[k]) == 0) continue;  // wave-uniform
         if (done[k]) continue;
         const float dy = xy.y - (float)(c.py + 4 * k);
         const float power = splat_power(co, dx, dy);
@@ -470,7 +506,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
   float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
 
   const unsigned row_lane = lane & 15;  // lane within its DPP row of 16
-  const WaveRect wr = wave_rect(c.px, c.py);
+  const int my_rect = SUB == 16 ? (int)(threadIdx.x >> 6) : 0;  // this wave's rectangle in the reach masks
   for (int r = 0; r < rounds; ++r) {
     __syncthreads();
     int n;
@@ -483,7 +519,8 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = r * BATCH + s;
-        if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P);
+        if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P,
+                                           c.rows);
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
@@ -491,7 +528,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
       n = min(BATCH, todo_total - r * BATCH);
     }
     for (int j = 0; j < n; ++j) {
-      if (outside(L.box[j], wr)) continue;  // no pixel of this wave can pass the alpha test
+      if (!reaches(L.mask[j], my_rect)) continue;  // no pixel of this wave can pass the alpha test
       uint32_t contributor;
       if constexpr (CELLS) {
         contributor = s_pos[j];
@@ -578,7 +615,6 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   float T[4], T_final[4], g0[4], g1[4], g2[4], bg_dot[4];
   float acc0[4], acc1[4], acc2[4], last_alpha[4], lc0[4], lc1[4], lc2[4];
   uint32_t last_contributor[4];
-  WaveRect strip[4];
   uint32_t m = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -601,7 +637,6 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
     T[k] = T_final[k];
     acc0[k] = acc1[k] = acc2[k] = 0.f;
     last_alpha[k] = lc0[k] = lc1[k] = lc2[k] = 0.f;
-    strip[k] = wave_rect(c.px, py);
     m = max(m, last_contributor[k]);
   }
   for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
@@ -621,7 +656,8 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = r * BATCH + s;
-        if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P);
+        if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P,
+                                           c.rows);
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
@@ -629,7 +665,8 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       n = min(BATCH, todo_total - r * BATCH);
     }
     for (int j = 0; j < n; ++j) {
-      const float4 box = L.box[j];
+      const uint32_t reach = __builtin_amdgcn_readfirstlane(L.mask[j]);
+      if (reach == 0u) continue;
       uint32_t contributor;
       if constexpr (CELLS) {
         contributor = s_pos[j];
@@ -644,7 +681,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       bool any = false;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        if (outside(box, strip[k])) continue;  // wave-uniform
+        if (!((reach >> k) & 1u)) continue;  // wave-uniform
         const float dx = xy.x - pxf, dy = xy.y - (float)(c.py + 4 * k);
         const float power = splat_power(co, dx, dy);
         const float G = __expf(power);
