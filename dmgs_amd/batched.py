@@ -249,8 +249,8 @@ class BatchedRenderer:
         return (int(self.instance_count().item()) & 0xFFFFFFFF) > self.K_cap
 
     def cell_lists(self):
-        """(ranges (cells, 2), sorted cell ids, sorted values) of the last
-        forward as CPU tensors; a value is entry id | tile mask << 28."""
+        """(ranges (cells, 2), sorted keys, entry ids) of the last forward as
+        CPU tensors; a key is cell id | tile mask << 16 (cells of 4x4 tiles)."""
         offs = (N.c_size_t * 8)()
         N.check(self.lib.dmgs_batched_offsets(self.B, self.P, self.total_tiles, self.total_pixels, self.K_cap, offs),
                 "dmgs_batched_offsets")

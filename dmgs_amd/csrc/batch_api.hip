@@ -105,12 +105,12 @@ struct ListLayout {
 
 int check_batch(const dmgs_batch* b) {
   if (!b) return bfail(DMGS_EINVAL, "batch is null");
-  // entry ids share 32 bits with the 4-bit tile mask of the cell lists
-  if (b->num_views <= 0 || b->num_gaussians <= 0 ||
-      (int64_t)b->num_views * b->num_gaussians > (int64_t)DMGS_CELL_ID_MASK + 1)
-    return bfail(DMGS_EINVAL, "invalid batch: %d views x %lld Gaussians (at most 2^28 entries)", b->num_views,
+  if (b->num_views <= 0 || b->num_gaussians <= 0 || (int64_t)b->num_views * b->num_gaussians > (int64_t)0x7fffffff)
+    return bfail(DMGS_EINVAL, "invalid batch: %d views x %lld Gaussians", b->num_views,
                  (long long)b->num_gaussians);
-  if (b->total_cells <= 0 || b->total_cells > b->total_tiles) return bfail(DMGS_EINVAL, "invalid cell total");
+  // cell ids share a key with the 16-bit tile mask of the cell lists
+  if (b->total_cells <= 0 || b->total_cells > b->total_tiles || b->total_cells > (1 << DMGS_CELL_KEY_BITS))
+    return bfail(DMGS_EINVAL, "invalid cell total (at most 65536 cells)");
   if (b->capacity < 0 || b->capacity > (int64_t)0xffffffff) return bfail(DMGS_EINVAL, "invalid instance capacity");
   if (b->total_tiles <= 0 || b->total_pixels <= 0) return bfail(DMGS_EINVAL, "invalid view totals");
   if (b->sh_degree < 0 || b->sh_degree > 3) return bfail(DMGS_EINVAL, "sh_degree must be in [0, 3]");
@@ -158,6 +158,7 @@ dmgs::RenderArgs render_args(const dmgs_batch* b, const Ctx& c) {
   r.tile_view = b->tile_view;
   r.ranges = at<uint2>(b->state, c.S.ranges);
   r.point_list = at<uint32_t>(b->lists, c.L.vals_a);
+  r.list_keys = at<uint32_t>(b->lists, c.L.tkeys_a);
   r.splats = at<float4>(b->state, c.S.splats);
   r.final_T = at<float>(b->state, c.S.final_T);
   r.n_contrib = at<uint32_t>(b->state, c.S.n_contrib);
@@ -185,7 +186,8 @@ int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles
     d.cell_base = (int32_t)cells;
     d.pixel_base = pixels;
     tiles += (int64_t)d.grid_x * d.grid_y;
-    cells += (int64_t)((d.grid_x + 1) / 2) * ((d.grid_y + 1) / 2);
+    const int E = 1 << DMGS_CELL_SHIFT;
+    cells += (int64_t)((d.grid_x + E - 1) / E) * ((d.grid_y + E - 1) / E);
     pixels += (int64_t)d.width * d.height;
     if (tiles > (int64_t)0x7fffffff) return bfail(DMGS_EINVAL, "too many tiles");
   }
@@ -223,8 +225,7 @@ size_t dmgs_batched_backward_size(int32_t B, int64_t P) {
 
 int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, int64_t capacity,
                          size_t offsets[8]) {
-  if (B <= 0 || P <= 0 || total_tiles <= 0 || total_pixels <= 0 || capacity < 0 ||
-      (int64_t)B * P > (int64_t)DMGS_CELL_ID_MASK + 1)
+  if (B <= 0 || P <= 0 || total_tiles <= 0 || total_pixels <= 0 || capacity < 0 || (int64_t)B * P > (int64_t)0x7fffffff)
     return bfail(DMGS_EINVAL, "invalid batch sizes");
   if (!offsets) return bfail(DMGS_EINVAL, "offsets is null");
   const size_t N = (size_t)B * P;
@@ -283,7 +284,8 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     dmgs::radix_sort_pairs(d.tile_keys, d.values, at<uint32_t>(b->lists, c.L.tkeys_b),
                            at<uint32_t>(b->lists, c.L.vals_b), K, bits_for((uint32_t)b->total_cells),
                            at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev);
-    dmgs::launch_tile_ranges(d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells, st, d.K_dev);
+    dmgs::launch_tile_ranges(d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells, st, d.K_dev,
+                             DMGS_CELL_KEY_MASK);
     if ((rc = bcheck("batch binning", st))) return rc;
   }
   if (stages & DMGS_STAGE_BLEND) {

@@ -74,26 +74,33 @@ __device__ __forceinline__ void tile_rect(float2 p, int r, int gx, int gy, int2&
 }
 
 // --- binning cells of the multi-view lists --------------------------------
-// The multi-view path bins each splat into cells of 2 x 2 blend tiles and
-// tags every (splat, cell) instance with a 4-bit mask of the cell's tiles the
-// splat's tile rectangle covers (bit (ty & 1) * 2 + (tx & 1)).  A tile's
-// list is its cell's list filtered by its bit, in the same order, so it
-// equals the per-tile list of the reference binning.
-#define DMGS_CELL_MASK_SHIFT 28
-#define DMGS_CELL_ID_MASK 0x0FFFFFFFu
-__device__ __forceinline__ int cells_x(int gx) { return (gx + 1) >> 1; }
+// The multi-view path bins each splat into cells of 4 x 4 blend tiles (64 x 64
+// pixels).  An instance's key is  cell id | tile mask << 16,  the mask
+// marking which of the cell's 16 tiles (bit (ty & 3) * 4 + (tx & 3)) the
+// splat's tile rectangle covers; sorting on the low 16 bits only carries the
+// mask along.  A tile's list is its cell's list filtered by its bit, in the
+// same order, so it equals the per-tile list of the reference binning.
+#define DMGS_CELL_SHIFT 2          // log2 of the cell edge in tiles
+#define DMGS_CELL_KEY_BITS 16      // cell id bits of a key (at most 65536 cells)
+#define DMGS_CELL_KEY_MASK 0xFFFFu
+__device__ __forceinline__ int cells_x(int gx) { return (gx + (1 << DMGS_CELL_SHIFT) - 1) >> DMGS_CELL_SHIFT; }
 // Number of cells a non-empty tile rectangle [rmin, rmax) touches.
 __device__ __forceinline__ int cell_count(int2 rmin, int2 rmax) {
-  return (((rmax.x - 1) >> 1) - (rmin.x >> 1) + 1) * (((rmax.y - 1) >> 1) - (rmin.y >> 1) + 1);
+  return (((rmax.x - 1) >> DMGS_CELL_SHIFT) - (rmin.x >> DMGS_CELL_SHIFT) + 1) *
+         (((rmax.y - 1) >> DMGS_CELL_SHIFT) - (rmin.y >> DMGS_CELL_SHIFT) + 1);
 }
-// Mask of the tiles of cell (cx, cy) inside the tile rectangle [rmin, rmax).
+// Mask of the tiles of cell (cx, cy) inside the tile rectangle [rmin, rmax):
+// the covered columns and rows of the cell, as an outer product of bit runs.
 __device__ __forceinline__ uint32_t cell_mask(int2 rmin, int2 rmax, int cx, int cy) {
+  constexpr int E = 1 << DMGS_CELL_SHIFT;
+  const int x0 = max(rmin.x - cx * E, 0), x1 = min(rmax.x - cx * E, E);
+  const int y0 = max(rmin.y - cy * E, 0), y1 = min(rmax.y - cy * E, E);
+  if (x1 <= x0 || y1 <= y0) return 0u;
+  const uint32_t row = ((1u << (x1 - x0)) - 1u) << x0;  // columns of one cell row
   uint32_t m = 0;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int tx = 2 * cx + (q & 1), ty = 2 * cy + (q >> 1);
-    if (tx >= rmin.x && tx < rmax.x && ty >= rmin.y && ty < rmax.y) m |= 1u << q;
-  }
+  for (int r = 0; r < E; ++r)
+    if (r >= y0 && r < y1) m |= row << (r * E);
   return m;
 }
 

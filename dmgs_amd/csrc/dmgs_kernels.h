@@ -53,7 +53,7 @@ void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, 
 
 // Emit (tile, splat) pairs in depth order.  Single view: grid_x/grid_y give
 // the tile grid.  Several views: entry g belongs to view g / per_view, the
-// pairs are (binning cell, splat | tile mask << 28) with cells numbered
+// pairs are (binning cell | tile mask << 16, splat) with cells numbered
 // globally from that view's cell_base.
 struct DupArgs {
   int P;                        // depth-sorted entries
@@ -72,8 +72,9 @@ struct DupArgs {
 };
 size_t duplicate_scratch_words(size_t K);
 void launch_duplicate(const DupArgs& a, hipStream_t s);
+// key_mask selects the tile / cell id bits of a key (the rest is carried data).
 void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s,
-                        const uint32_t* K_dev = nullptr);
+                        const uint32_t* K_dev = nullptr, uint32_t key_mask = 0xFFFFFFFFu);
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 
 // ---- multi-view per-Gaussian stages ----
@@ -117,6 +118,7 @@ struct RenderArgs {
   const int32_t* tile_view;
   const uint2* ranges;
   const uint32_t* point_list;
+  const uint32_t* list_keys;  // multi-view: sorted keys (cell | tile mask << 16) beside point_list
   const float4* splats;
   const float* bg;
   float* out_color;      // (3, H, W)

@@ -160,7 +160,7 @@ struct TileCtx {
   size_t plane;        // W * H of the view
   size_t img;          // offset of the view's colour planes
   float bg0, bg1, bg2;
-  uint32_t bit;        // multi-view: this tile's bit in its cell's instance tags
+  uint32_t bit;        // multi-view: this tile's bit in its cell's key masks
   uint2 range;         // single view: the tile's list; multi-view: its cell's list
   RowRects rows;       // the work-group's pixel rectangles: 16x4 strips (SUB 16) or the 8x8 block
 };
@@ -180,8 +180,9 @@ __device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
     lt = c.tile - v.tile_base;
     gx = v.grid_x;
     const int ltx = lt % gx, lty = lt / gx;
-    list = v.cell_base + (lty >> 1) * cells_x(gx) + (ltx >> 1);
-    c.bit = 1u << (((lty & 1) << 1) | (ltx & 1));
+    constexpr int E = 1 << DMGS_CELL_SHIFT;
+    list = v.cell_base + (lty >> DMGS_CELL_SHIFT) * cells_x(gx) + (ltx >> DMGS_CELL_SHIFT);
+    c.bit = 1u << ((lty & (E - 1)) * E + (ltx & (E - 1)));
     c.W = v.W;
     c.H = v.H;
     c.bg0 = v.bg[0];
@@ -220,25 +221,27 @@ __device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
   return c;
 }
 
-// Stage one batch of a cell list (multi-view): visit BATCH list positions —
-// forward: base, base+1, ... below len; backward: top-1, top-2, ... down to 0
-// — keep the instances tagged with this tile's bit and stage them in visit
-// order.  s_pos receives each staged instance's list position.  Returns the
-// number staged (workgroup-uniform); ends with a barrier.
+// Stage the next instances of this tile from its cell list (multi-view):
+// visit list positions from `cursor` — forward: cursor, cursor+1, ... below
+// limit; backward: cursor-1, cursor-2, ... down to 0 — min(NT, BATCH) at a
+// time, keep those whose key holds this tile's bit, stage them in visit
+// order, and go on while another chunk surely fits in the BATCH slots.
+// Advances cursor; returns the number staged (workgroup-uniform); ends with
+// a barrier.  s_pos receives each staged instance's list position.
 template <int NT, int BATCH, bool BACKWARD>
 __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_pos, uint32_t* s_wc,
-                                                const RenderArgs& a, const TileCtx& c, int start, int limit) {
+                                                const RenderArgs& a, const TileCtx& c, int& cursor, int limit) {
   constexpr int NW = NT / DMGS_WAVE;
+  constexpr int CH = NT < BATCH ? NT : BATCH;  // positions per chunk
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = lanemask_lt();
   int run = 0;
-#pragma unroll
-  for (int k = 0; k < (BATCH + NT - 1) / NT; ++k) {
-    const int s = k * NT + (int)threadIdx.x;
-    const int q = BACKWARD ? start - 1 - s : start + s;
-    const bool valid = s < BATCH && (BACKWARD ? q >= 0 : q < limit);
-    const uint32_t val = valid ? a.point_list[c.range.x + q] : 0u;
-    const bool member = valid && ((val >> DMGS_CELL_MASK_SHIFT) & c.bit);
+  do {
+    const int s = (int)threadIdx.x;
+    const int q = BACKWARD ? cursor - 1 - s : cursor + s;
+    const bool valid = s < CH && (BACKWARD ? q >= 0 : q < limit);
+    const uint32_t key = valid ? a.list_keys[c.range.x + q] : 0u;
+    const bool member = valid && ((key >> DMGS_CELL_KEY_BITS) & c.bit);
     const uint64_t bal = __ballot(member);
     if (lane == 0) s_wc[wave] = (uint32_t)__popcll(bal);
     __syncthreads();
@@ -251,12 +254,13 @@ __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_
     }
     if (member) {
       const int slot = before + (int)__popcll(bal & lt);
-      stage_splat(L, slot, a.splats, val & DMGS_CELL_ID_MASK, a.P, c.rows);
+      stage_splat(L, slot, a.splats, a.point_list[c.range.x + q], a.P, c.rows);
       s_pos[slot] = (uint32_t)q;
     }
     run += total;
+    cursor = BACKWARD ? cursor - CH : cursor + CH;
     __syncthreads();
-  }
+  } while (run + CH <= BATCH && (BACKWARD ? cursor > 0 : cursor < limit));
   return run;
 }
 
@@ -337,9 +341,10 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a
   uint32_t last_contributor = 0;
   const int my_rect = SUB == 16 ? (int)(threadIdx.x >> 6) : 0;  // this wave's rectangle in the reach masks
 
-  for (int base = 0; base < len; base += BATCH) {
+  int cursor = 0;
+  while (cursor < len) {
     if (__syncthreads_count(done) == NT) break;
-    const int n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, base, len);
+    const int n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, cursor, len);
     for (int j = 0; !done && j < n; ++j) {
       if (!reaches(L.mask[j], my_rect)) continue;
       const float2 xy = L.xy[j];
@@ -398,19 +403,21 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
     C0[k] = C1[k] = C2[k] = 0.f;
     last_contributor[k] = 0;
   }
-  const int rounds = (len + BATCH - 1) / BATCH;
-  for (int r = 0; r < rounds; ++r) {
+  int cursor = 0;
+  while (cursor < len) {
     if (__syncthreads_count(done[0] & done[1] & done[2] & done[3]) == NT) break;
+    const int base = cursor;
     int n;
     if constexpr (CELLS) {
-      n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, r * BATCH, len);
+      n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, cursor, len);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
-        const int pos = r * BATCH + s;
+        const int pos = base + s;
         if (pos < len) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
       }
       __syncthreads();
-      n = min(BATCH, len - r * BATCH);
+      n = min(BATCH, len - base);
+      cursor += n;
     }
     for (int j = 0; j < n; ++j) {
       const uint32_t reach = __builtin_amdgcn_readfirstlane(L.mask[j]);
@@ -421,7 +428,7 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
       if constexpr (CELLS) {
         position = s_pos[j];
       } else {
-        position = (uint32_t)(r * BATCH + j);
+        position = (uint32_t)(base + j);
       }
       const float dx = xy.x - pxf;
 #pragma unroll
@@ -499,7 +506,6 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
   uint32_t tile_max = 0;
   for (int w = 0; w < NW; ++w) tile_max = max(tile_max, s_max[w]);
   const int todo_total = min((int)(c.range.y - c.range.x), (int)tile_max);
-  const int rounds = (todo_total + BATCH - 1) / BATCH;
 
   float T = T_final;
   float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
@@ -507,25 +513,27 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
 
   const unsigned row_lane = lane & 15;  // lane within its DPP row of 16
   const int my_rect = SUB == 16 ? (int)(threadIdx.x >> 6) : 0;  // this wave's rectangle in the reach masks
-  for (int r = 0; r < rounds; ++r) {
+  int cursor = todo_total;  // list positions below cursor are still to do
+  while (cursor > 0) {
     __syncthreads();
+    const int top = cursor;
     int n;
     if constexpr (CELLS) {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
-      n = stage_cell_batch<NT, BATCH, true>(L, s_pos, s_wc, a, c, todo_total - r * BATCH, 0);
+      n = stage_cell_batch<NT, BATCH, true>(L, s_pos, s_wc, a, c, cursor, 0);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
-        const int pos = r * BATCH + s;
-        if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P,
-                                           c.rows);
+        const int pos = top - 1 - s;
+        if (pos >= 0) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
       __syncthreads();
-      n = min(BATCH, todo_total - r * BATCH);
+      n = min(BATCH, top);
+      cursor -= n;
     }
     for (int j = 0; j < n; ++j) {
       if (!reaches(L.mask[j], my_rect)) continue;  // no pixel of this wave can pass the alpha test
@@ -533,7 +541,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
       if constexpr (CELLS) {
         contributor = s_pos[j];
       } else {
-        contributor = (uint32_t)(todo_total - 1 - (r * BATCH + j));
+        contributor = (uint32_t)(top - 1 - j);
       }
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
@@ -641,28 +649,29 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   }
   for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
   const int todo_total = min((int)(c.range.y - c.range.x), (int)m);
-  const int rounds = (todo_total + BATCH - 1) / BATCH;
 
   const unsigned row_lane = lane & 15;
-  for (int r = 0; r < rounds; ++r) {
+  int cursor = todo_total;  // list positions below cursor are still to do
+  while (cursor > 0) {
     __syncthreads();
+    const int top = cursor;
     int n;
     if constexpr (CELLS) {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
-      n = stage_cell_batch<NT, BATCH, true>(L, s_pos, s_wc, a, c, todo_total - r * BATCH, 0);
+      n = stage_cell_batch<NT, BATCH, true>(L, s_pos, s_wc, a, c, cursor, 0);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
-        const int pos = r * BATCH + s;
-        if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + todo_total - pos - 1], a.P,
-                                           c.rows);
+        const int pos = top - 1 - s;
+        if (pos >= 0) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
       __syncthreads();
-      n = min(BATCH, todo_total - r * BATCH);
+      n = min(BATCH, top);
+      cursor -= n;
     }
     for (int j = 0; j < n; ++j) {
       const uint32_t reach = __builtin_amdgcn_readfirstlane(L.mask[j]);
@@ -671,7 +680,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       if constexpr (CELLS) {
         contributor = s_pos[j];
       } else {
-        contributor = (uint32_t)(todo_total - 1 - (r * BATCH + j));
+        contributor = (uint32_t)(top - 1 - j);
       }
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];

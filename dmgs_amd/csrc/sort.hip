@@ -405,13 +405,13 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
           const int r = __float_as_int(a.splats[(size_t)DMGS_SPLAT_F4 * g + 2].z);
           tile_rect(make_float2(s0.x, s0.y), r, gx, a.grid_y, rmin, rmax);
         }
-        if (v) {  // emit per binning cell of 2x2 tiles
+        if (v) {  // emit per binning cell of 4x4 tiles
           rx = (uint32_t)rmin.x | ((uint32_t)rmax.x << 16);
           ry = (uint32_t)rmin.y | ((uint32_t)rmax.y << 16);
           tb = (uint32_t)v->cell_base;
-          x0 = rmin.x >> 1;
-          y0 = rmin.y >> 1;
-          w = max(1, ((rmax.x - 1) >> 1) + 1 - x0);
+          x0 = rmin.x >> DMGS_CELL_SHIFT;
+          y0 = rmin.y >> DMGS_CELL_SHIFT;
+          w = max(1, ((rmax.x - 1) >> DMGS_CELL_SHIFT) + 1 - x0);
           gx = cells_x(gx);
         } else {
           x0 = rmin.x;
@@ -445,15 +445,15 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
       const uint32_t wj = (uint32_t)s_w[lo];
       const uint32_t row = local / wj;
       const uint32_t cx = (uint32_t)s_x0[lo] + (local - row * wj), cy = (uint32_t)s_y0[lo] + row;
-      a.tile_keys[o] = s_tb[lo] + cy * (uint32_t)s_gx[lo] + cx;
-      uint32_t val = s_g[lo];
-      if (a.views) {
+      uint32_t key = s_tb[lo] + cy * (uint32_t)s_gx[lo] + cx;
+      if (a.views) {  // cell id | tile mask << 16
         const uint32_t rx = s_rx[lo], ry = s_ry[lo];
         const int2 rmin = make_int2((int)(rx & 0xFFFFu), (int)(ry & 0xFFFFu));
         const int2 rmax = make_int2((int)(rx >> 16), (int)(ry >> 16));
-        val |= cell_mask(rmin, rmax, (int)cx, (int)cy) << DMGS_CELL_MASK_SHIFT;
+        key |= cell_mask(rmin, rmax, (int)cx, (int)cy) << DMGS_CELL_KEY_BITS;
       }
-      a.values[o] = val;
+      a.tile_keys[o] = key;
+      a.values[o] = s_g[lo];
     }
     const bool more = win_end < hi_slot && first + 256 < P;
     __syncthreads();
@@ -484,22 +484,22 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
 // predecessor: a key that differs from the previous one closes the previous
 // tile's range and opens its own.
 __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, uint2* __restrict__ ranges,
-                                   uint32_t num_tiles, const uint32_t* __restrict__ K_dev) {
+                                   uint32_t num_tiles, const uint32_t* __restrict__ K_dev, uint32_t key_mask) {
   if (K_dev) K = min(K, (size_t)*K_dev);
   const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (i0 >= K) return;
   uint32_t k[4];
   if (i0 + 4 <= K) {
     const uint4 q = *reinterpret_cast<const uint4*>(keys + i0);
-    k[0] = q.x;
-    k[1] = q.y;
-    k[2] = q.z;
-    k[3] = q.w;
+    k[0] = q.x & key_mask;
+    k[1] = q.y & key_mask;
+    k[2] = q.z & key_mask;
+    k[3] = q.w & key_mask;
   } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) k[j] = (i0 + j < K) ? keys[i0 + j] : 0u;
+    for (int j = 0; j < 4; ++j) k[j] = (i0 + j < K) ? keys[i0 + j] & key_mask : 0u;
   }
-  uint32_t prev = (i0 == 0) ? 0u : keys[i0 - 1];
+  uint32_t prev = (i0 == 0) ? 0u : keys[i0 - 1] & key_mask;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const size_t i = i0 + j;
@@ -519,11 +519,11 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, 
 }
 
 void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s,
-                        const uint32_t* K_dev) {
+                        const uint32_t* K_dev, uint32_t key_mask) {
   launch_fill_u32((uint32_t*)ranges, (size_t)num_tiles * 2, 0u, s);
   const size_t threads = (K + 3) / 4;
   if (K) hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, tile_keys,
-                            K, ranges, (uint32_t)num_tiles, K_dev);
+                            K, ranges, (uint32_t)num_tiles, K_dev, key_mask);
 }
 
 }  // namespace dmgs

@@ -235,15 +235,16 @@ typedef struct dmgs_view {
   int32_t width, height;
   int32_t grid_x, grid_y;   /* filled */
   int32_t tile_base;        /* filled */
-  int32_t cell_base;        /* filled: first binning cell (2x2 tiles) of the view */
+  int32_t cell_base;        /* filled: first binning cell (4x4 tiles) of the view */
   int64_t pixel_base;       /* filled */
 } dmgs_view;
 
 /* Host-side: fill the derived fields of B host records and return the tile,
  * binning-cell and pixel totals of the batch.  The batched lists bin splats
- * into cells of 2x2 tiles (32x32 pixels), each instance tagged with the
- * cell's tiles it covers; a tile's list is its cell's list filtered by its
- * tag, i.e. exactly the reference per-tile list. */
+ * into cells of 4x4 tiles (64x64 pixels), each instance's key carrying a
+ * mask of the cell's tiles it covers; a tile's list is its cell's list
+ * filtered by its bit, i.e. exactly the reference per-tile list.  At most
+ * 65536 cells per batch. */
 int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles, int32_t* total_cells,
                                int64_t* total_pixels);
 /* Host-side: tile_view[t] = view owning global tile t (host array of total_tiles). */
@@ -265,8 +266,8 @@ size_t dmgs_batched_backward_size(int32_t B, int64_t P);
  * count of the last forward (uint32), [2] cell ranges (uint2 per global
  * cell), [3] final_T, [4] last contributor (uint32 per packed pixel: one past
  * the position of its last contributing instance in its cell's list),
- * [5] splat records; inside `lists`: [6] sorted cell ids, [7] sorted values
- * (entry id | tile mask << 28). */
+ * [5] splat records; inside `lists`: [6] sorted keys (cell id | tile mask
+ * << 16), [7] the entry ids beside them. */
 int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, int64_t capacity,
                          size_t offsets[8]);
 

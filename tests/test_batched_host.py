@@ -40,7 +40,7 @@ def test_prepare_views_and_tile_map():
         assert d.focal_x == pytest.approx(w / (2 * math.tan(cams[v].FoVx / 2)), rel=1e-6)
         assert d.focal_y == pytest.approx(h / (2 * math.tan(cams[v].FoVy / 2)), rel=1e-6)
         tiles += d.grid_x * d.grid_y
-        cells += ((w + 31) // 32) * ((h + 31) // 32)  # cells of 2x2 tiles = 32x32 pixels
+        cells += ((w + 63) // 64) * ((h + 63) // 64)  # cells of 4x4 tiles = 64x64 pixels
         pixels += w * h
     assert (tt.value, tc.value, tp.value) == (tiles, cells, pixels)
     tv = torch.full((tiles,), -1, dtype=torch.int32)
@@ -77,7 +77,7 @@ def test_workspace_and_offsets():
     assert len(set(offs[:6])) == 6
     assert offs[6] < offs[7] < lib.dmgs_batched_lists_size(K)
     assert lib.dmgs_batched_offsets(B, 0, T, X, K, offs) == -1
-    assert lib.dmgs_batched_offsets(B, 1 << 27, T, X, K, offs) == -1  # more than 2^28 entries
+    assert lib.dmgs_batched_offsets(B, 1 << 30, T, X, K, offs) == -1  # more than 2^31 - 1 entries
 
 
 def _desc(**kw):
@@ -110,7 +110,9 @@ def test_entry_points_validate_before_launch():
     # too many entries for 32-bit indices
     assert lib.dmgs_batched_forward(_desc(num_views=4096, num_gaussians=1 << 20), fake, fake, A, None) == -1
     assert "invalid batch" in N.last_error()
-    assert lib.dmgs_batched_forward(_desc(num_views=2, num_gaussians=(1 << 27) + 1), fake, fake, A, None) == -1
+    assert lib.dmgs_batched_forward(_desc(num_views=2, num_gaussians=1 << 30), fake, fake, A, None) == -1
+    assert lib.dmgs_batched_forward(_desc(total_tiles=1 << 20, total_cells=(1 << 16) + 1), fake, fake, A,
+                                    None) == -1  # cell ids are 16 bits
     assert lib.dmgs_batched_forward(_desc(sh_degree=4), fake, fake, A, None) == -1
     assert "sh_degree" in N.last_error()
     assert lib.dmgs_batched_forward(_desc(state=0x1004), fake, fake, A, None) == -1

@@ -172,7 +172,7 @@ def test_instance_count_and_overflow():
     K = r.count_instances(params)
     # the (splat, cell) instances of the drop-in path's tile rectangles: a
     # rectangle [x0, x1) x [y0, y1) of 16-pixel tiles touches the cells
-    # x0 // 2 .. (x1 - 1) // 2 (and the same in y)
+    # x0 // 4 .. (x1 - 1) // 4 (and the same in y)
     from dmgs_amd import rasterizer as RZ
     from tests.helpers import settings_from_cam, workspace_views
     m = model.to("cuda")
@@ -184,7 +184,7 @@ def test_instance_count_and_overflow():
         wv = workspace_views(st, w, h)
         vis = radii.cpu().numpy() > 0
         x0, y0, x1, y1 = OR.rect_f32(wv["xy"][vis], radii.cpu().numpy()[vis], w, h)
-        Ks.append(int((((x1 - 1) // 2 - x0 // 2 + 1) * ((y1 - 1) // 2 - y0 // 2 + 1)).sum()))
+        Ks.append(int((((x1 - 1) // 4 - x0 // 4 + 1) * ((y1 - 1) // 4 - y0 // 4 + 1)).sum()))
         Ktiles.append(st["K"])
     assert K == sum(Ks)
     assert K < sum(Ktiles)
@@ -274,11 +274,12 @@ def test_cell_lists_equal_reference_tile_lists(sizes):
     torch.cuda.synchronize()
     ranges, keys, vals = r.cell_lists()
     ranges, keys, vals = ranges.numpy(), keys.numpy().view(np.uint32), vals.numpy().view(np.uint32)
-    assert np.all(np.diff(keys.astype(np.int64)) >= 0), "cell ids must be sorted"
+    assert np.all(np.diff((keys & 0xFFFF).astype(np.int64)) >= 0), "cell ids must be sorted"
     o = r._offsets[5]
     splats = r.state[o:o + 48 * r.B * r.P].view(torch.float32).view(r.B * r.P, 12).cpu().numpy()
-    ids = vals & 0x0FFFFFFF
-    tags = vals >> 28
+    ids = vals
+    tags = keys >> 16
+    keys = keys & 0xFFFF
     all_radii = r.radii().cpu().numpy()  # splat records of culled entries are not written
     checked = 0
     for v in range(r.B):
@@ -291,11 +292,11 @@ def test_cell_lists_equal_reference_tile_lists(sizes):
         depth = splats[sl, 9].copy()
         _, ovals, oranges = OR.bin_gaussians(xy, radii, depth, W, H)
         gx, gy = d.grid_x, d.grid_y
-        cgx = (gx + 1) // 2
+        cgx = (gx + 3) // 4  # cells of 4x4 tiles
         for t in range(gx * gy):
             tx, ty = t % gx, t // gx
-            cell = d.cell_base + (ty // 2) * cgx + tx // 2
-            bit = 1 << ((ty & 1) * 2 + (tx & 1))
+            cell = d.cell_base + (ty // 4) * cgx + tx // 4
+            bit = 1 << ((ty & 3) * 4 + (tx & 3))
             s0, s1 = ranges[cell]
             mine = ids[s0:s1][(tags[s0:s1] & bit) != 0].astype(np.int64) - v * r.P
             ref = ovals[oranges[t, 0]:oranges[t, 1]]
