@@ -89,29 +89,49 @@ __global__ __launch_bounds__(kScanThreads) void scan_spine_kernel(uint32_t* __re
   }
 }
 
+// LDS tile index with one padding word per 32 (thread-contiguous reads of 16
+// consecutive words then hit different banks).
+__device__ __forceinline__ int scan_pad(int p) { return p + (p >> 5); }
+
+// The tile moves between HBM and registers through LDS: global loads and
+// stores stay fully coalesced (consecutive lanes, consecutive words) while
+// each thread scans 16 consecutive elements.
 template <bool kExclusive>
 __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* __restrict__ in,
                                                                  uint32_t* __restrict__ out, size_t n,
                                                                  const uint32_t* __restrict__ spine) {
   __shared__ uint32_t s_wave[kScanThreads / 64];
-  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+  __shared__ uint32_t tile[kScanTile + kScanTile / 32];
+  const size_t base = (size_t)blockIdx.x * kScanTile;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const int p = i * kScanThreads + (int)threadIdx.x;
+    const size_t idx = base + p;
+    tile[scan_pad(p)] = idx < n ? in[idx] : 0u;
+  }
+  __syncthreads();
   uint32_t v[kScanItems];
   uint32_t tsum = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    const size_t idx = base + i;
-    v[i] = idx < n ? in[idx] : 0u;
+    v[i] = tile[scan_pad((int)threadIdx.x * kScanItems + i)];
     tsum += v[i];
   }
   uint32_t total;
-  uint32_t run = spine[blockIdx.x] + block_exclusive_scan(tsum, s_wave, total);
+  uint32_t run = spine[blockIdx.x] + block_exclusive_scan(tsum, s_wave, total);  // ends with a barrier
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    const size_t idx = base + i;
     const uint32_t x = v[i];
     if (!kExclusive) run += x;
-    if (idx < n) out[idx] = run;
+    tile[scan_pad((int)threadIdx.x * kScanItems + i)] = run;
     if (kExclusive) run += x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    const int p = i * kScanThreads + (int)threadIdx.x;
+    const size_t idx = base + p;
+    if (idx < n) out[idx] = tile[scan_pad(p)];
   }
 }
 
@@ -155,10 +175,22 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
   cnt[threadIdx.x] = 0;
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * (kSortThreads * ITEMS);
+  const int lane = threadIdx.x & 63;
+  // Lanes holding the same digit are found with one ballot per digit bit and
+  // counted by one LDS atomic from the group's highest lane: skewed digits
+  // (e.g. the exponent byte of depth keys) no longer serialise 64 atomics.
 #pragma unroll 4
   for (int i = 0; i < ITEMS; ++i) {
     const size_t idx = base + (size_t)i * kSortThreads + threadIdx.x;
-    if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & mask], 1u);
+    const bool valid = idx < n;
+    const uint32_t d = valid ? (keys[idx] >> shift) & mask : 0u;
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < nbits; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    if (valid && lane == 63 - __clzll(peers)) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
   }
   __syncthreads();
   if (threadIdx.x <= mask) hist[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
