@@ -305,25 +305,35 @@ class BatchedLoss:
         self.targets = renderer.new_images()
         for v, t in enumerate(targets):
             renderer.image(self.targets, v).copy_(t.to(device=dev, dtype=torch.float32))
-        self.ws = [torch.empty(self.lib.dmgs_dssim_l1_workspace_size(3, h, w), dtype=torch.uint8, device=dev)
-                   for h, w in renderer.sizes]
+        # consecutive views of one size are back to back in the packed buffers:
+        # each such run is one grouped launch pair (one loss per view)
+        self.runs = []
+        for v, hw in enumerate(renderer.sizes):
+            if self.runs and self.runs[-1][2] == hw:
+                self.runs[-1][1] = v + 1
+            else:
+                self.runs.append([v, v + 1, hw])
+        self.ws = [torch.empty(self.lib.dmgs_dssim_l1_workspace_size(3 * (v1 - v0), h, w), dtype=torch.uint8,
+                               device=dev) for v0, v1, (h, w) in self.runs]
         self.losses = torch.zeros(renderer.B, 3, dtype=torch.float32, device=dev)
-        self.one = torch.ones(1, dtype=torch.float32, device=dev)
+        self.ones = torch.ones(renderer.B, dtype=torch.float32, device=dev)
 
     def forward(self, images):
+        """(B, 3) device tensor: (loss, l1, ssim) per view."""
         r, s = self.r, N.stream_handle(self.r.device)
-        for v, (h, w) in enumerate(r.sizes):
-            N.check(self.lib.dmgs_dssim_l1_forward(N.ptr(r.image(images, v)), N.ptr(r.image(self.targets, v)), 3,
-                                                   h, w, self.lam, N.ptr(self.ws[v]), N.ptr(self.losses[v]), s),
-                    "dmgs_dssim_l1_forward")
+        for ws, (v0, v1, (h, w)) in zip(self.ws, self.runs):
+            N.check(self.lib.dmgs_dssim_l1_forward_groups(
+                N.ptr(r.image(images, v0)), N.ptr(r.image(self.targets, v0)), v1 - v0, 3, h, w, self.lam, N.ptr(ws),
+                N.ptr(self.losses[v0]), s), "dmgs_dssim_l1_forward_groups")
         return self.losses
 
     def backward(self, images, dimages):
+        """dL/dimages of the summed per-view loss (every view's dloss = 1)."""
         r, s = self.r, N.stream_handle(self.r.device)
-        for v, (h, w) in enumerate(r.sizes):
-            N.check(self.lib.dmgs_dssim_l1_backward(N.ptr(r.image(images, v)), N.ptr(r.image(self.targets, v)), 3,
-                                                    h, w, self.lam, N.ptr(self.ws[v]), N.ptr(self.one),
-                                                    N.ptr(r.image(dimages, v)), s), "dmgs_dssim_l1_backward")
+        for ws, (v0, v1, (h, w)) in zip(self.ws, self.runs):
+            N.check(self.lib.dmgs_dssim_l1_backward_groups(
+                N.ptr(r.image(images, v0)), N.ptr(r.image(self.targets, v0)), v1 - v0, 3, h, w, self.lam, N.ptr(ws),
+                N.ptr(self.ones[v0:v1]), N.ptr(r.image(dimages, v0)), s), "dmgs_dssim_l1_backward_groups")
         return dimages
 
 

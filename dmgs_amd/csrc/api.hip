@@ -510,32 +510,45 @@ size_t dmgs_dssim_l1_workspace_size(int32_t C, int32_t H, int32_t W) {
   return (C <= 0 || H <= 0 || W <= 0) ? 0 : SsimLayout(C, H, W).total;
 }
 
-int dmgs_dssim_l1_forward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
-                          void* ws, float* loss_out, dmgs_stream_t stream) {
+int dmgs_dssim_l1_forward_groups(const float* img, const float* gt, int32_t G, int32_t Cg, int32_t H, int32_t W,
+                                 float lambda_dssim, void* ws, float* loss_out, dmgs_stream_t stream) {
   if (!img || !gt || !ws || !loss_out) return fail(DMGS_EINVAL, "null pointer");
-  if (C <= 0 || H <= 0 || W <= 0 || (int64_t)C * H * W > ((int64_t)1 << 31))
-    return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d", C, H, W);
+  if (G <= 0 || Cg <= 0 || H <= 0 || W <= 0 || (int64_t)G * Cg * H * W > ((int64_t)1 << 31))
+    return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d x %d", G, Cg, H, W);
+  if (misaligned(ws)) return fail(DMGS_EINVAL, "ws must be 256-byte aligned");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  SsimLayout L(C, H, W);
+  SsimLayout L(G * Cg, H, W);
   float w[11];
   ssim_window(w);
-  // the final reduction writes (loss, l1, ssim) straight to loss_out (no copy node when captured)
-  dmgs::launch_dssim_forward(img, gt, C, H, W, w, lambda_dssim, at<float>(ws, L.coef), at<float>(ws, L.partials),
-                             loss_out, st);
+  // the final reduction writes (loss, l1, ssim) per group straight to loss_out (no copy node when captured)
+  dmgs::launch_dssim_forward(img, gt, G * Cg, Cg, H, W, w, lambda_dssim, at<float>(ws, L.coef),
+                             at<float>(ws, L.partials), loss_out, st);
   return check_launch("dssim forward", st, false);
+}
+
+int dmgs_dssim_l1_backward_groups(const float* img, const float* gt, int32_t G, int32_t Cg, int32_t H, int32_t W,
+                                  float lambda_dssim, const void* ws, const float* dloss, float* dimg,
+                                  dmgs_stream_t stream) {
+  if (!img || !gt || !ws || !dloss || !dimg) return fail(DMGS_EINVAL, "null pointer");
+  if (G <= 0 || Cg <= 0 || H <= 0 || W <= 0 || (int64_t)G * Cg * H * W > ((int64_t)1 << 31))
+    return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d x %d", G, Cg, H, W);
+  if (misaligned(ws)) return fail(DMGS_EINVAL, "ws must be 256-byte aligned");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  SsimLayout L(G * Cg, H, W);
+  float w[11];
+  ssim_window(w);
+  dmgs::launch_dssim_backward(img, gt, at<float>(ws, L.coef), G * Cg, Cg, H, W, w, lambda_dssim, dloss, dimg, st);
+  return check_launch("dssim backward", st, false);
+}
+
+int dmgs_dssim_l1_forward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
+                          void* ws, float* loss_out, dmgs_stream_t stream) {
+  return dmgs_dssim_l1_forward_groups(img, gt, 1, C, H, W, lambda_dssim, ws, loss_out, stream);
 }
 
 int dmgs_dssim_l1_backward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
                            const void* ws, const float* dloss, float* dimg, dmgs_stream_t stream) {
-  if (!img || !gt || !ws || !dloss || !dimg) return fail(DMGS_EINVAL, "null pointer");
-  if (C <= 0 || H <= 0 || W <= 0 || (int64_t)C * H * W > ((int64_t)1 << 31))
-    return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d", C, H, W);
-  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  SsimLayout L(C, H, W);
-  float w[11];
-  ssim_window(w);
-  dmgs::launch_dssim_backward(img, gt, at<float>(ws, L.coef), C, H, W, w, lambda_dssim, dloss, dimg, st);
-  return check_launch("dssim backward", st, false);
+  return dmgs_dssim_l1_backward_groups(img, gt, 1, C, H, W, lambda_dssim, ws, dloss, dimg, stream);
 }
 
 }  // extern "C"

@@ -213,9 +213,10 @@ void launch_faces_in_frustum(int F, const float* verts, const int32_t* faces, co
 
 // ---- fused L1 + SSIM loss ----
 size_t dssim_partials(int C, int H, int W);
-void launch_dssim_forward(const float* x, const float* y, int C, int H, int W, const float* w11, float lam,
-                          float* coef, float* partials, float* out, hipStream_t s);
-void launch_dssim_backward(const float* x, const float* y, const float* coef, int C, int H, int W, const float* w11,
-                           float lam, const float* dloss, float* dx, hipStream_t s);
+// C planes in C / group loss groups of `group` consecutive planes each.
+void launch_dssim_forward(const float* x, const float* y, int C, int group, int H, int W, const float* w11,
+                          float lam, float* coef, float* partials, float* out, hipStream_t s);
+void launch_dssim_backward(const float* x, const float* y, const float* coef, int C, int group, int H, int W,
+                           const float* w11, float lam, const float* dloss, float* dx, hipStream_t s);
 
 }  // namespace dmgs

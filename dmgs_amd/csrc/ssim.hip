@@ -26,8 +26,10 @@ struct Window11 {
   float w[11];
 };
 
+// One loss per group of `group` consecutive planes (group = C: one loss over
+// the whole image; group = 3: one per RGB view of a batch).
 __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                        int H, int W, Window11 g, float lam,
+                                                        int H, int W, int group, Window11 g, float lam,
                                                         float* __restrict__ coef, float* __restrict__ partials) {
   __shared__ float sx[kP][kP + 1], sy[kP][kP + 1];
   __shared__ float hq[5][kP][kT + 1];
@@ -85,8 +87,8 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
   if (inside) {
     S = (A1 * A2) / (B1 * B2);
     l1 = fabsf(sx[ly + kR][lx + kR] - sy[ly + kR][lx + kR]);
-    // dL/dS for the mean over C*H*W pixels
-    const float dS = -lam / (float)(gridDim.z * plane);
+    // dL/dS for the mean over the group's group*H*W pixels
+    const float dS = -lam / (float)((size_t)group * plane);
     const float inv = 1.f / (B1 * B2);
     const float dmu1 = (2.f * mu2 * A2 - 2.f * mu2 * A1) * inv - S * (2.f * mu1 / B1 - 2.f * mu1 / B2);
     const float ds11 = -S / B2;
@@ -112,10 +114,14 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
   }
 }
 
-// Single workgroup: loss = (1-lam) * L1 + lam * (1 - SSIM); out = (loss, l1, ssim).
+// One workgroup per loss group (its partials are contiguous: `nblocks`
+// blocks per group): loss = (1-lam) * L1 + lam * (1 - SSIM);
+// out[3 g ..] = (loss, l1, ssim).
 __global__ __launch_bounds__(256) void dssim_reduce_kernel(const float* __restrict__ partials, int nblocks,
                                                            float inv_n, float lam, float* __restrict__ out) {
   __shared__ float red[2][4];
+  partials += (size_t)2 * nblocks * blockIdx.x;
+  out += 3 * blockIdx.x;
   float s = 0.f, l = 0.f;
   for (int i = threadIdx.x; i < nblocks; i += 256) {
     s += partials[2 * i];
@@ -140,8 +146,8 @@ __global__ __launch_bounds__(256) void dssim_reduce_kernel(const float* __restri
 
 __global__ __launch_bounds__(256) void dssim_bwd_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                         const float* __restrict__ coef, int H, int W, int C,
-                                                        Window11 g, float lam, const float* __restrict__ dloss,
-                                                        float* __restrict__ dx) {
+                                                        int group, Window11 g, float lam,
+                                                        const float* __restrict__ dloss, float* __restrict__ dx) {
   __shared__ float sa[3][kP][kP + 1];
   __shared__ float hq[3][kP][kT + 1];
   const int c = blockIdx.z;
@@ -187,27 +193,27 @@ __global__ __launch_bounds__(256) void dssim_bwd_kernel(const float* __restrict_
   const float xv = x[o], yv = y[o];
   const float d = xv - yv;
   const float sgn = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
-  const float g_l1 = (1.f - lam) * sgn / (float)all;
-  dx[o] = dloss[0] * (b0 + 2.f * xv * b1 + yv * b2 + g_l1);
+  const float g_l1 = (1.f - lam) * sgn / (float)((size_t)group * plane);
+  dx[o] = dloss[c / group] * (b0 + 2.f * xv * b1 + yv * b2 + g_l1);
 }
 
-void launch_dssim_forward(const float* x, const float* y, int C, int H, int W, const float* w11, float lam,
-                          float* coef, float* partials, float* out, hipStream_t s) {
+void launch_dssim_forward(const float* x, const float* y, int C, int group, int H, int W, const float* w11,
+                          float lam, float* coef, float* partials, float* out, hipStream_t s) {
   Window11 g;
   for (int k = 0; k < 11; ++k) g.w[k] = w11[k];
   const dim3 grid((W + kT - 1) / kT, (H + kT - 1) / kT, C);
-  hipLaunchKernelGGL(dssim_fwd_kernel, grid, dim3(256), 0, s, x, y, H, W, g, lam, coef, partials);
-  const int nb = (int)(grid.x * grid.y * grid.z);
-  hipLaunchKernelGGL(dssim_reduce_kernel, dim3(1), dim3(256), 0, s, (const float*)partials, nb,
-                     1.0f / (float)((size_t)C * H * W), lam, out);
+  hipLaunchKernelGGL(dssim_fwd_kernel, grid, dim3(256), 0, s, x, y, H, W, group, g, lam, coef, partials);
+  const int per_group = (int)(grid.x * grid.y) * group;
+  hipLaunchKernelGGL(dssim_reduce_kernel, dim3(C / group), dim3(256), 0, s, (const float*)partials, per_group,
+                     1.0f / (float)((size_t)group * H * W), lam, out);
 }
 
-void launch_dssim_backward(const float* x, const float* y, const float* coef, int C, int H, int W, const float* w11,
-                           float lam, const float* dloss, float* dx, hipStream_t s) {
+void launch_dssim_backward(const float* x, const float* y, const float* coef, int C, int group, int H, int W,
+                           const float* w11, float lam, const float* dloss, float* dx, hipStream_t s) {
   Window11 g;
   for (int k = 0; k < 11; ++k) g.w[k] = w11[k];
   const dim3 grid((W + kT - 1) / kT, (H + kT - 1) / kT, C);
-  hipLaunchKernelGGL(dssim_bwd_kernel, grid, dim3(256), 0, s, x, y, coef, H, W, C, g, lam, dloss, dx);
+  hipLaunchKernelGGL(dssim_bwd_kernel, grid, dim3(256), 0, s, x, y, coef, H, W, C, group, g, lam, dloss, dx);
 }
 
 size_t dssim_partials(int C, int H, int W) {

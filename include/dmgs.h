@@ -202,6 +202,16 @@ int dmgs_dssim_l1_forward(const float* img, const float* gt, int32_t C, int32_t 
                           void* ws, float* loss_out, dmgs_stream_t stream);
 int dmgs_dssim_l1_backward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
                            const void* ws, const float* dloss, float* dimg, dmgs_stream_t stream);
+/* The same for G images of Cg channels each, stored back to back (G*Cg planes
+ * of H x W; workspace dmgs_dssim_l1_workspace_size(G*Cg, H, W)): one loss per
+ * image, (loss, l1, ssim) at loss_out[3g..3g+2]; the backward scales image
+ * g's gradient by dloss[g].  Two launches for all G images (the per-view loss
+ * of a batch of equally sized views). */
+int dmgs_dssim_l1_forward_groups(const float* img, const float* gt, int32_t G, int32_t Cg, int32_t H, int32_t W,
+                                 float lambda_dssim, void* ws, float* loss_out, dmgs_stream_t stream);
+int dmgs_dssim_l1_backward_groups(const float* img, const float* gt, int32_t G, int32_t Cg, int32_t H, int32_t W,
+                                  float lambda_dssim, const void* ws, const float* dloss, float* dimg,
+                                  dmgs_stream_t stream);
 
 /* ---- batched views --------------------------------------------------------
  * Replaces the per-view loop of the training / evaluation scripts

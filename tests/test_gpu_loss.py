@@ -38,3 +38,36 @@ def test_fused_dssim_l1_identical_images():
     x = torch.rand(3, 40, 40, generator=torch.Generator().manual_seed(3)).cuda()
     v = L.fused_dssim_l1(x, x.clone(), 0.2)
     assert abs(v.item()) < 1e-5   # L1 = 0 and SSIM = 1
+
+
+def test_grouped_loss_equals_per_image_calls():
+    """G images in one grouped launch pair give each image's own loss and
+    gradient (scaled by its dloss) as separate single-image calls."""
+    import ctypes
+    from dmgs_amd import _native as N
+    print("case: grouped loss, G=4 images 3x70x90", flush=True)
+    lib = N.load()
+    g = torch.Generator().manual_seed(12)
+    G, H, W = 4, 70, 90
+    img = torch.rand(G * 3, H, W, generator=g).cuda()
+    gt = torch.rand(G * 3, H, W, generator=g).cuda()
+    dloss = torch.tensor([1.0, 0.5, 2.0, -1.0]).cuda()
+    ws = torch.empty(lib.dmgs_dssim_l1_workspace_size(G * 3, H, W), dtype=torch.uint8, device="cuda")
+    out = torch.empty(G, 3, device="cuda")
+    dimg = torch.empty_like(img)
+    s = N.stream_handle()
+    N.check(lib.dmgs_dssim_l1_forward_groups(N.ptr(img), N.ptr(gt), G, 3, H, W, 0.2, N.ptr(ws), N.ptr(out), s), "f")
+    N.check(lib.dmgs_dssim_l1_backward_groups(N.ptr(img), N.ptr(gt), G, 3, H, W, 0.2, N.ptr(ws), N.ptr(dloss),
+                                              N.ptr(dimg), s), "b")
+    for k in range(G):
+        x = img[3 * k:3 * k + 3].contiguous()
+        y = gt[3 * k:3 * k + 3].contiguous()
+        ws1 = torch.empty(lib.dmgs_dssim_l1_workspace_size(3, H, W), dtype=torch.uint8, device="cuda")
+        o1 = torch.empty(3, device="cuda")
+        d1 = torch.empty_like(x)
+        N.check(lib.dmgs_dssim_l1_forward(N.ptr(x), N.ptr(y), 3, H, W, 0.2, N.ptr(ws1), N.ptr(o1), s), "f1")
+        N.check(lib.dmgs_dssim_l1_backward(N.ptr(x), N.ptr(y), 3, H, W, 0.2, N.ptr(ws1), N.ptr(dloss[k:k + 1]),
+                                           N.ptr(d1), s), "b1")
+        torch.cuda.synchronize()
+        assert torch.allclose(out[k], o1, rtol=1e-6, atol=1e-7), (k, out[k], o1)
+        assert torch.allclose(dimg[3 * k:3 * k + 3], d1, rtol=1e-6, atol=1e-12), k
