@@ -25,10 +25,10 @@
 #define DMGS_SPLAT_F4 3
 
 // Per-Gaussian accumulator written by the blend backward with atomics:
-// 16 floats (64 B, one cache line) per Gaussian.
+// 12 floats (48 B: three 16-byte vectors, 9 used) per Gaussian.
 //   [0] dL/dmean2D.x (NDC)  [1] dL/dmean2D.y  [2] dL/dconic.a  [3] dL/dconic.b (half)
 //   [4] dL/dconic.c  [5] dL/dopacity  [6..8] dL/dcolour rgb
-#define DMGS_GACC_STRIDE 16
+#define DMGS_GACC_STRIDE 12
 #define DMGS_GACC_USED 9
 
 namespace dmgs {

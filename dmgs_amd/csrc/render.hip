@@ -495,8 +495,8 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
     g1 = dp[c.plane + pid];
     g2 = dp[2 * c.plane + pid];
   }
-  const float bg_dot = c.bg0 * g0 + c.bg1 * g1 + c.bg2 * g2;
-  const float ddelx_dx = 0.5f * (float)c.W, ddely_dy = 0.5f * (float)c.H;
+  const float bg_term = -T_final * (c.bg0 * g0 + c.bg1 * g1 + c.bg2 * g2);  // -T_final <bg, dL/dpixel>
+  const float ddelx_dx = 0.5f * (float)c.W, ddely_dy = 0.5f * (float)c.H;  // d pixel / d NDC
 
   // Entries at list positions >= max(last contributor) of the workgroup contribute nothing.
   uint32_t m = last_contributor;
@@ -569,16 +569,17 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
         v[8] = dchannel_dcolor * g2;
         dL_dalpha *= T;
         last_alpha = alpha;
-        dL_dalpha += -T_final * inv_one_m * bg_dot;
+        dL_dalpha += bg_term * inv_one_m;
         const float dL_dG = co.w * dL_dalpha;
         const float gdx = G * dx, gdy = G * dy;
         const float dG_ddelx = -gdx * co.x - gdy * co.y;
         const float dG_ddely = -gdy * co.z - gdx * co.y;
-        v[0] = dL_dG * dG_ddelx * ddelx_dx;
-        v[1] = dL_dG * dG_ddely * ddely_dy;
-        v[2] = -0.5f * gdx * dx * dL_dG;
-        v[3] = -0.5f * gdx * dy * dL_dG;
-        v[4] = -0.5f * gdy * dy * dL_dG;
+        // constant factors (W/2, H/2, -1/2) are applied once per splat at the flush
+        v[0] = dL_dG * dG_ddelx;
+        v[1] = dL_dG * dG_ddely;
+        v[2] = gdx * dx * dL_dG;
+        v[3] = gdx * dy * dL_dG;
+        v[4] = gdy * dy * dL_dG;
         v[5] = G * dL_dalpha;
       } else {
 #pragma unroll
@@ -593,7 +594,8 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
     // flush the batch: consecutive lanes cover the 9 values of one splat
     for (int i = threadIdx.x; i < n * DMGS_GACC_USED; i += NT) {
       const int slot = i / DMGS_GACC_USED, k = i - slot * DMGS_GACC_USED;
-      const float val = s_acc[slot][k];
+      const float scale = k == 0 ? ddelx_dx : (k == 1 ? ddely_dy : (k < 5 ? -0.5f : 1.0f));
+      const float val = s_acc[slot][k] * scale;
       if (val != 0.f) {
         const uint32_t gid = __float_as_uint(L.rgb_id[slot].w);
         unsafeAtomicAdd(a.gacc + (size_t)gid * DMGS_GACC_STRIDE + k, val);
@@ -618,9 +620,9 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   const TileCtx c = tile_ctx<16>(a);  // pixel k = 0 of this lane: (lx, ly), ly in 0..3
   const unsigned lane = threadIdx.x & 63;
   const float pxf = (float)c.px;
-  const float ddelx_dx = 0.5f * (float)c.W, ddely_dy = 0.5f * (float)c.H;
+  const float ddelx_dx = 0.5f * (float)c.W, ddely_dy = 0.5f * (float)c.H;  // d pixel / d NDC
 
-  float T[4], T_final[4], g0[4], g1[4], g2[4], bg_dot[4];
+  float T[4], T_final[4], g0[4], g1[4], g2[4], bg_term[4];
   float acc0[4], acc1[4], acc2[4], last_alpha[4], lc0[4], lc1[4], lc2[4];
   uint32_t last_contributor[4];
   uint32_t m = 0;
@@ -641,7 +643,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       g1[k] = dp[c.plane + pid];
       g2[k] = dp[2 * c.plane + pid];
     }
-    bg_dot[k] = c.bg0 * g0[k] + c.bg1 * g1[k] + c.bg2 * g2[k];
+    bg_term[k] = -T_final[k] * (c.bg0 * g0[k] + c.bg1 * g1[k] + c.bg2 * g2[k]);
     T[k] = T_final[k];
     acc0[k] = acc1[k] = acc2[k] = 0.f;
     last_alpha[k] = lc0[k] = lc1[k] = lc2[k] = 0.f;
@@ -713,16 +715,17 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
           v[8] += dchannel_dcolor * g2[k];
           dL_dalpha *= T[k];
           last_alpha[k] = alpha;
-          dL_dalpha += -T_final[k] * inv_one_m * bg_dot[k];
+          dL_dalpha += bg_term[k] * inv_one_m;
           const float dL_dG = co.w * dL_dalpha;
           const float gdx = G * dx, gdy = G * dy;
           const float dG_ddelx = -gdx * co.x - gdy * co.y;
           const float dG_ddely = -gdy * co.z - gdx * co.y;
-          v[0] += dL_dG * dG_ddelx * ddelx_dx;
-          v[1] += dL_dG * dG_ddely * ddely_dy;
-          v[2] += -0.5f * gdx * dx * dL_dG;
-          v[3] += -0.5f * gdx * dy * dL_dG;
-          v[4] += -0.5f * gdy * dy * dL_dG;
+          // constant factors (W/2, H/2, -1/2) are applied once per splat at the flush
+          v[0] += dL_dG * dG_ddelx;
+          v[1] += dL_dG * dG_ddely;
+          v[2] += gdx * dx * dL_dG;
+          v[3] += gdx * dy * dL_dG;
+          v[4] += gdy * dy * dL_dG;
           v[5] += G * dL_dalpha;
         }
       }
@@ -733,7 +736,8 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
     __syncthreads();
     for (int i = threadIdx.x; i < n * DMGS_GACC_USED; i += NT) {
       const int slot = i / DMGS_GACC_USED, q = i - slot * DMGS_GACC_USED;
-      const float val = s_acc[slot][q];
+      const float scale = q == 0 ? ddelx_dx : (q == 1 ? ddely_dy : (q < 5 ? -0.5f : 1.0f));
+      const float val = s_acc[slot][q] * scale;
       if (val != 0.f) {
         const uint32_t gid = __float_as_uint(L.rgb_id[slot].w);
         unsafeAtomicAdd(a.gacc + (size_t)gid * DMGS_GACC_STRIDE + q, val);
