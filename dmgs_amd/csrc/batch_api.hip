@@ -68,9 +68,9 @@ int bits_for(uint32_t n) {
 // K-independent state of a batch: per-entry splats / keys / order / radii /
 // scan, the depth-sort scratch, per-tile ranges and per-pixel blend state.
 struct StateLayout {
-  size_t splats, keys_a, keys_b, order_a, order_b, rects, rects_sorted, radii, offsets, gscratch, ranges, final_T,
-      n_contrib, total;
-  StateLayout(size_t N, size_t T, size_t X) {
+  size_t splats, keys_a, keys_b, order_a, order_b, rects, rects_sorted, radii, offsets, gscratch, wave_counts,
+      ranges, final_T, n_contrib, total;
+  StateLayout(size_t N, size_t T, size_t X, size_t wave_words) {
     size_t o = 0;
     splats = o; o += align_up(N * DMGS_SPLAT_F4 * 16);
     keys_a = o; o += align_up(N * 4);
@@ -82,6 +82,7 @@ struct StateLayout {
     radii = o; o += align_up(N * 4);
     offsets = o; o += align_up(N * 4);
     gscratch = o; o += align_up(std::max(dmgs::radix_scratch_words(N), dmgs::scan_scratch_words(N)) * 4);
+    wave_counts = o; o += align_up(wave_words * 4);
     ranges = o; o += align_up(T * 8);
     final_T = o; o += align_up(X * 4);
     n_contrib = o; o += align_up(X * 4);
@@ -126,7 +127,9 @@ struct Ctx {
   StateLayout S;
   ListLayout L;
   explicit Ctx(const dmgs_batch* b)
-      : N((size_t)b->num_views * b->num_gaussians), S(N, (size_t)b->total_tiles, (size_t)b->total_pixels),
+      : N((size_t)b->num_views * b->num_gaussians),
+        S(N, (size_t)b->total_tiles, (size_t)b->total_pixels,
+          dmgs::mv_wave_count_words((int)b->num_gaussians, b->num_views)),
         L((size_t)b->capacity) {}
   uint32_t* count(const dmgs_batch* b) const { return at<uint32_t>(b->state, S.offsets) + (N - 1); }
 };
@@ -142,8 +145,9 @@ dmgs::MVArgs mv_args(const dmgs_batch* b, const Ctx& c, const float* params) {
   a.splats = at<float4>(b->state, c.S.splats);
   a.radii = at<int32_t>(b->state, c.S.radii);
   a.rects = at<uint2>(b->state, c.S.rects);
-  a.depth_keys = at<uint32_t>(b->state, c.S.keys_a);
+  a.depth_keys = at<uint32_t>(b->state, c.S.keys_b);  // compacted into keys_a / order_a before the sort
   a.order = at<uint32_t>(b->state, c.S.order_a);
+  a.wave_counts = at<uint32_t>(b->state, c.S.wave_counts);
   a.gacc = static_cast<float*>(b->bwd_ws);
   return a;
 }
@@ -210,7 +214,8 @@ int dmgs_batched_tile_view(const dmgs_view* views, int32_t B, int32_t* tile_view
 
 size_t dmgs_batched_state_size(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels) {
   if (B <= 0 || P <= 0 || total_tiles <= 0 || total_pixels <= 0) return 0;
-  return StateLayout((size_t)B * P, (size_t)total_tiles, (size_t)total_pixels).total;
+  return StateLayout((size_t)B * P, (size_t)total_tiles, (size_t)total_pixels,
+                     dmgs::mv_wave_count_words((int)P, B)).total;
 }
 
 size_t dmgs_batched_lists_size(int64_t capacity) {
@@ -229,7 +234,7 @@ int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t tota
     return bfail(DMGS_EINVAL, "invalid batch sizes");
   if (!offsets) return bfail(DMGS_EINVAL, "offsets is null");
   const size_t N = (size_t)B * P;
-  StateLayout S(N, (size_t)total_tiles, (size_t)total_pixels);
+  StateLayout S(N, (size_t)total_tiles, (size_t)total_pixels, dmgs::mv_wave_count_words((int)P, B));
   ListLayout Ls((size_t)capacity);
   offsets[6] = Ls.tkeys_a;
   offsets[7] = Ls.vals_a;
@@ -259,9 +264,15 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
   if (stages & DMGS_STAGE_GEOMETRY) {
     dmgs::launch_mv_preprocess(a, st);
     uint32_t* gs = at<uint32_t>(b->state, c.S.gscratch);
-    dmgs::radix_sort_pairs(a.depth_keys, a.order, at<uint32_t>(b->state, c.S.keys_b),
-                           at<uint32_t>(b->state, c.S.order_b), N, 32, gs, st);
-    dmgs::launch_gather_rects(a.rects, a.order, at<uint2>(b->state, c.S.rects_sorted), offs, N, st);
+    // visible entries, in entry order, become the depth sort's input
+    const size_t nw = dmgs::mv_wave_count_words((int)b->num_gaussians, b->num_views);
+    dmgs::launch_inclusive_scan(a.wave_counts, a.wave_counts, nw, gs, st);
+    const uint32_t* V_dev = a.wave_counts + (nw - 1);
+    uint32_t* keys = at<uint32_t>(b->state, c.S.keys_a);
+    dmgs::launch_mv_compact(a, a.depth_keys, keys, a.order, st);
+    dmgs::radix_sort_pairs(keys, a.order, at<uint32_t>(b->state, c.S.keys_b), at<uint32_t>(b->state, c.S.order_b),
+                           N, 32, gs, st, V_dev);
+    dmgs::launch_gather_rects(a.rects, a.order, at<uint2>(b->state, c.S.rects_sorted), offs, N, st, V_dev);
     dmgs::launch_inclusive_scan(offs, offs, N, gs, st);
     if ((rc = bcheck("batch preprocess / depth sort / scan", st))) return rc;
   }

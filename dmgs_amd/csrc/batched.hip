@@ -54,21 +54,31 @@ __device__ __forceinline__ void load_mv(const MVArgs& a, int g, MVGaussian& G) {
   cov3d_from_sR(G.s, G.R, G.c3);
 }
 
+// One wave's count of visible entries per view, at wave_counts[(v * blocks +
+// block) * 4 + wave]: scanned, they give every visible entry its slot in the
+// compacted, entry-ordered depth-sort input.
+__device__ __forceinline__ void count_visible(const MVArgs& a, int v, bool visible) {
+  const uint64_t bal = __ballot(visible);
+  if ((threadIdx.x & 63) == 0)
+    a.wave_counts[((size_t)v * gridDim.x + blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6)] =
+        (uint32_t)__popcll(bal);
+}
+
 __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.P) return;
+  const bool live = g < a.P;  // every lane stays for the per-view ballots
   MVGaussian G;
-  load_mv(a, g, G);
+  if (live) load_mv(a, g, G);
   for (int v = 0; v < a.B; ++v) {
     const ViewDesc& vd = a.views[v];
     const ViewParams vp = view_params(vd);
     const size_t i = (size_t)v * a.P + g;
-    a.order[i] = (uint32_t)i;
     SplatGeom o;
-    if (!project_splat(G.p, G.c3, vp, o)) {
+    const bool visible = live && project_splat(G.p, G.c3, vp, o);
+    count_visible(a, v, visible);
+    if (!live) continue;
+    if (!visible) {  // only the radius is read for a culled entry
       a.radii[i] = 0;
-      a.rects[i] = make_uint2(0u, 0u);
-      a.depth_keys[i] = 0xFFFFFFFFu;
       continue;
     }
     float3 dir = make_float3(G.p.x - vd.campos[0], G.p.y - vd.campos[1], G.p.z - vd.campos[2]);
@@ -83,6 +93,29 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
     a.radii[i] = o.radius;
     a.rects[i] = pack_rect(o.rmin, o.rmax);  // binning reads rectangles in depth order
     a.depth_keys[i] = __float_as_uint(o.depth);
+  }
+}
+
+// Compact the visible entries, in entry order, into (depth key, entry id)
+// pairs: the depth sort then runs over the visible entries only, with the
+// same stable tie order as over all entries.  wave_counts holds the
+// inclusive scan of the preprocess's per-wave counts.
+__global__ __launch_bounds__(256) void mv_compact_kernel(MVArgs a, const uint32_t* __restrict__ depth_keys,
+                                                         uint32_t* __restrict__ keys_out,
+                                                         uint32_t* __restrict__ ids_out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = g < a.P;
+  const uint64_t lt = lanemask_lt();
+  for (int v = 0; v < a.B; ++v) {
+    const size_t i = (size_t)v * a.P + g;
+    const bool visible = live && a.radii[i] > 0;
+    const uint64_t bal = __ballot(visible);
+    const size_t w = ((size_t)v * gridDim.x + blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (visible) {
+      const uint32_t slot = (w == 0 ? 0u : a.wave_counts[w - 1]) + (uint32_t)__popcll(bal & lt);
+      keys_out[slot] = depth_keys[i];
+      ids_out[slot] = (uint32_t)i;
+    }
   }
 }
 
@@ -162,9 +195,14 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
 __global__ __launch_bounds__(256) void gather_rects_kernel(const uint2* __restrict__ rects,
                                                            const uint32_t* __restrict__ order,
                                                            uint2* __restrict__ rects_sorted,
-                                                           uint32_t* __restrict__ counts, size_t n) {
+                                                           uint32_t* __restrict__ counts, size_t n,
+                                                           const uint32_t* __restrict__ n_dev) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (n_dev && i >= *n_dev) {  // past the sorted entries: no instances
+    counts[i] = 0u;
+    return;
+  }
   const uint2 r = rects[order[i]];
   rects_sorted[i] = r;
   int2 rmin, rmax;
@@ -173,15 +211,23 @@ __global__ __launch_bounds__(256) void gather_rects_kernel(const uint2* __restri
 }
 
 void launch_gather_rects(const uint2* rects, const uint32_t* order, uint2* rects_sorted, uint32_t* counts, size_t n,
-                         hipStream_t s) {
+                         hipStream_t s, const uint32_t* n_dev) {
   if (n == 0) return;
   hipLaunchKernelGGL(gather_rects_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rects, order,
-                     rects_sorted, counts, n);
+                     rects_sorted, counts, n, n_dev);
 }
 
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s) {
   if (a.P <= 0 || a.B <= 0) return;
   hipLaunchKernelGGL(mv_preprocess_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+}
+
+size_t mv_wave_count_words(int P, int B) { return (size_t)B * ((P + 255) / 256) * 4; }
+
+void launch_mv_compact(const MVArgs& a, const uint32_t* depth_keys, uint32_t* keys_out, uint32_t* ids_out,
+                       hipStream_t s) {
+  if (a.P <= 0 || a.B <= 0) return;
+  hipLaunchKernelGGL(mv_compact_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a, depth_keys, keys_out, ids_out);
 }
 
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s) {

@@ -93,15 +93,22 @@ struct MVArgs {
   float4* splats;            // (B*P) records
   int32_t* radii;            // (B*P)
   uint2* rects;              // (B*P) tile rectangle, pack_rect; 0 when culled
-  uint32_t* depth_keys;      // (B*P)
+  uint32_t* depth_keys;      // (B*P), culled entries left unwritten
   uint32_t* order;           // (B*P)
+  uint32_t* wave_counts;     // mv_wave_count_words(P, B): visible entries per (view, block, wave)
   float* gacc;               // (B*P, 16) blend-backward accumulator; zero between backward passes
   float* grads;              // (59, P)
 };
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s);
-// rects_sorted[i] = rects[order[i]], counts[i] = its number of binning cells.
+size_t mv_wave_count_words(int P, int B);
+// After an inclusive scan of wave_counts: visible entries in entry order as
+// (depth key, entry id) pairs; their number is wave_counts[last].
+void launch_mv_compact(const MVArgs& a, const uint32_t* depth_keys, uint32_t* keys_out, uint32_t* ids_out,
+                       hipStream_t s);
+// rects_sorted[i] = rects[order[i]], counts[i] = its number of binning cells
+// (0 for i >= *n_dev when given).
 void launch_gather_rects(const uint2* rects, const uint32_t* order, uint2* rects_sorted, uint32_t* counts, size_t n,
-                         hipStream_t s);
+                         hipStream_t s, const uint32_t* n_dev = nullptr);
 // Reads the accumulator rows of visible entries and leaves them zeroed.
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s);
 
