@@ -42,6 +42,7 @@ class DmgsBatch(ctypes.Structure):
         ("num_gaussians", c_int64), ("scale_modifier", c_float), ("variant", c_int32), ("total_tiles", c_int32),
         ("total_cells", c_int32), ("total_pixels", c_int64), ("capacity", c_int64),
         ("state", c_void_p), ("lists", c_void_p), ("bwd_ws", c_void_p),
+        ("cell_mode", c_int32), ("max_view_cells", c_int32),
     ]
 
 
@@ -86,7 +87,10 @@ _SIGNATURES = {
     "dmgs_scan_scratch_size": (c_size_t, [c_int64]),
     "dmgs_inclusive_scan_u32": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "dmgs_batched_prepare_views": (c_int32, [ctypes.POINTER(DmgsView), c_int32, ctypes.POINTER(c_int32),
-                                             ctypes.POINTER(c_int32), ctypes.POINTER(c_int64)]),
+                                             ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
+                                             ctypes.POINTER(c_int64)]),
+    "dmgs_batched_list_offsets": (c_int32, [ctypes.POINTER(DmgsBatch), ctypes.POINTER(c_size_t),
+                                            ctypes.POINTER(c_size_t)]),
     "dmgs_batched_tile_view": (c_int32, [ctypes.POINTER(DmgsView), c_int32, c_void_p, c_int32]),
     "dmgs_batched_state_size": (c_size_t, [c_int32, c_int64, c_int32, c_int64]),
     "dmgs_batched_lists_size": (c_size_t, [c_int64]),

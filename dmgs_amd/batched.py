@@ -102,7 +102,7 @@ class BatchedRenderer:
     image_width, image_height); ``bg`` is one colour or one per view."""
 
     def __init__(self, cameras, bg, num_gaussians, sh_degree=3, scale_modifier=1.0, device="cuda",
-                 headroom=1.2, variant=0):
+                 headroom=1.2, variant=0, cell_mode=None):
         if not cameras:
             raise ValueError("BatchedRenderer needs at least one camera")
         self.lib = N.load()
@@ -114,10 +114,14 @@ class BatchedRenderer:
         self.headroom, self.variant = float(headroom), int(variant)
         bgs = bg if isinstance(bg, (list, tuple)) else [bg] * self.B
         self.views = (N.DmgsView * self.B)(*[_view_record(c, b) for c, b in zip(cameras, bgs)])
-        tt, tc, tp = N.c_int32(0), N.c_int32(0), N.c_int64(0)
+        tt, tc, mc, tp = N.c_int32(0), N.c_int32(0), N.c_int32(0), N.c_int64(0)
         N.check(self.lib.dmgs_batched_prepare_views(self.views, self.B, ctypes.byref(tt), ctypes.byref(tc),
-                                                    ctypes.byref(tp)), "dmgs_batched_prepare_views")
+                                                    ctypes.byref(mc), ctypes.byref(tp)), "dmgs_batched_prepare_views")
         self.total_tiles, self.total_cells, self.total_pixels = tt.value, tc.value, tp.value
+        self.max_view_cells = mc.value
+        # view-major lists (one cell-sort pass for views of up to 256 cells) where they apply
+        fits = self.B <= 16 and self.max_view_cells <= 4096
+        self.cell_mode = (1 if fits else 0) if cell_mode is None else int(cell_mode)
         tv = torch.empty(self.total_tiles, dtype=torch.int32)
         N.check(self.lib.dmgs_batched_tile_view(self.views, self.B, ctypes.c_void_p(tv.data_ptr()),
                                                 self.total_tiles), "dmgs_batched_tile_view")
@@ -135,7 +139,7 @@ class BatchedRenderer:
         n = self.lib.dmgs_batched_state_size(self.B, self.P, self.total_tiles, self.total_pixels)
         if n == 0:
             raise RuntimeError("dmgs_batched_state_size rejected the batch")
-        offs = (N.c_size_t * 8)()
+        offs = (N.c_size_t * 9)()
         N.check(self.lib.dmgs_batched_offsets(self.B, self.P, self.total_tiles, self.total_pixels, 0, offs),
                 "dmgs_batched_offsets")
         self.state = torch.empty(n, dtype=torch.uint8, device=self.device)
@@ -147,6 +151,7 @@ class BatchedRenderer:
         d.num_views, d.sh_degree, d.num_gaussians = self.B, self.sh_degree, self.P
         d.scale_modifier, d.variant = self.scale_modifier, self.variant
         d.total_tiles, d.total_cells, d.total_pixels = self.total_tiles, self.total_cells, self.total_pixels
+        d.cell_mode, d.max_view_cells = self.cell_mode, self.max_view_cells
         d.capacity = self.K_cap if capacity is None else capacity
         d.state = self.state.data_ptr()
         d.lists = None if self.lists is None else self.lists.data_ptr()
@@ -165,14 +170,18 @@ class BatchedRenderer:
 
     def count_instances_async(self, params, host_out):
         """Launch preprocess + depth sort + scan and copy the instance count
-        into ``host_out`` (pinned int32 tensor of one element) without
-        waiting; the count is valid once the current stream has reached it."""
+        into ``host_out[0]`` (pinned int32 tensor) without waiting, and the
+        status word (bit 0: see ``depth_clamped``) into ``host_out[1]`` when
+        it has two elements; valid once the current stream has reached it."""
         if self.state is None:
             self._alloc_state()
         self._check_params(params)
         N.check(self.lib.dmgs_batched_forward(ctypes.byref(self._desc(0)), N.ptr(params), None, STAGE_GEOMETRY,
                                               N.stream_handle(self.device)), "dmgs_batched_forward")
-        host_out.copy_(self.instance_count(), non_blocking=True)
+        host_out[0:1].copy_(self.instance_count(), non_blocking=True)
+        if host_out.numel() > 1:
+            o = self._offsets[8]
+            host_out[1:2].copy_(self.state[o:o + 4].view(torch.int32), non_blocking=True)
 
     def plan(self, params, K_cap=None, backward=True):
         """Size the instance lists for ``params`` (with headroom) and allocate
@@ -243,22 +252,36 @@ class BatchedRenderer:
         o = self._offsets[1]
         return self.state[o:o + 4].view(torch.int32)
 
+    def depth_clamped(self):
+        """True when the last view-major forward met a visible depth of 2^29
+        or more (its lists may misorder such far splats).  Reads back."""
+        o = self._offsets[8]
+        return self.cell_mode == 1 and (int(self.state[o:o + 4].view(torch.int32).item()) & 1) != 0
+
     def overflowed(self):
-        """True when the last forward needed more instances than planned
-        (its lists were truncated; re-plan and render again).  Reads back."""
+        """True when the last forward must be redone: it needed more
+        instances than planned (its lists were truncated; re-plan and render
+        again) or, in view-major mode, met a depth the view-major keys cannot
+        order (the renderer then switches to global cell lists; render
+        again).  Reads back."""
+        if self.depth_clamped():
+            self.cell_mode = 0
+            return True
         return (int(self.instance_count().item()) & 0xFFFFFFFF) > self.K_cap
 
     def cell_lists(self):
         """(ranges (cells, 2), sorted keys, entry ids) of the last forward as
-        CPU tensors; a key is cell id | tile mask << 16 (cells of 4x4 tiles)."""
-        offs = (N.c_size_t * 8)()
-        N.check(self.lib.dmgs_batched_offsets(self.B, self.P, self.total_tiles, self.total_pixels, self.K_cap, offs),
-                "dmgs_batched_offsets")
+        CPU tensors.  A key is cell | tile mask << 16 (cells of 4x4 tiles,
+        ``ranges`` indexed by global cell id); the cell is the global id in
+        cell mode 0 and (cell within the view | view << 12) in mode 1."""
+        ko, vo = N.c_size_t(0), N.c_size_t(0)
+        N.check(self.lib.dmgs_batched_list_offsets(ctypes.byref(self._desc()), ctypes.byref(ko), ctypes.byref(vo)),
+                "dmgs_batched_list_offsets")
         K = min(int(self.instance_count().item()) & 0xFFFFFFFF, self.K_cap)
         o = self._offsets[2]
         ranges = self.state[o:o + 8 * self.total_cells].view(torch.int32).view(-1, 2).cpu()
-        keys = self.lists[offs[6]:offs[6] + 4 * K].view(torch.int32).cpu()
-        vals = self.lists[offs[7]:offs[7] + 4 * K].view(torch.int32).cpu()
+        keys = self.lists[ko.value:ko.value + 4 * K].view(torch.int32).cpu()
+        vals = self.lists[vo.value:vo.value + 4 * K].view(torch.int32).cpu()
         return ranges, keys, vals
 
     def radii(self):

@@ -31,7 +31,7 @@ static_assert(offsetof(dmgs_view, pixel_base) == offsetof(dmgs::ViewDesc, pixel_
 static_assert(offsetof(dmgs_view, tanfovx) == offsetof(dmgs::ViewDesc, tanfovx), "dmgs_view layout");
 static_assert(offsetof(dmgs_view, width) == offsetof(dmgs::ViewDesc, W), "dmgs_view layout");
 static_assert(offsetof(dmgs_view, bg) == offsetof(dmgs::ViewDesc, bg), "dmgs_view layout");
-static_assert(sizeof(dmgs_batch) == 88 && offsetof(dmgs_batch, capacity) == 56, "dmgs_batch layout");
+static_assert(sizeof(dmgs_batch) == 96 && offsetof(dmgs_batch, capacity) == 56, "dmgs_batch layout");
 static_assert(offsetof(dmgs_view, cell_base) == offsetof(dmgs::ViewDesc, cell_base), "dmgs_view layout");
 
 namespace {
@@ -69,7 +69,7 @@ int bits_for(uint32_t n) {
 // scan, the depth-sort scratch, per-tile ranges and per-pixel blend state.
 struct StateLayout {
   size_t splats, keys_a, keys_b, order_a, order_b, rects, rects_sorted, radii, offsets, gscratch, wave_counts,
-      ranges, final_T, n_contrib, total;
+      status, ranges, final_T, n_contrib, total;
   StateLayout(size_t N, size_t T, size_t X, size_t wave_words) {
     size_t o = 0;
     splats = o; o += align_up(N * DMGS_SPLAT_F4 * 16);
@@ -83,6 +83,7 @@ struct StateLayout {
     offsets = o; o += align_up(N * 4);
     gscratch = o; o += align_up(std::max(dmgs::radix_scratch_words(N), dmgs::scan_scratch_words(N)) * 4);
     wave_counts = o; o += align_up(wave_words * 4);
+    status = o; o += align_up(4);
     ranges = o; o += align_up(T * 8);
     final_T = o; o += align_up(X * 4);
     n_contrib = o; o += align_up(X * 4);
@@ -114,6 +115,10 @@ int check_batch(const dmgs_batch* b) {
     return bfail(DMGS_EINVAL, "invalid cell total (at most 65536 cells)");
   if (b->capacity < 0 || b->capacity > (int64_t)0xffffffff) return bfail(DMGS_EINVAL, "invalid instance capacity");
   if (b->total_tiles <= 0 || b->total_pixels <= 0) return bfail(DMGS_EINVAL, "invalid view totals");
+  if (b->cell_mode != 0 && b->cell_mode != 1) return bfail(DMGS_EINVAL, "cell_mode must be 0 or 1");
+  if (b->cell_mode == 1 && (b->num_views > 16 || b->max_view_cells <= 0 || b->max_view_cells > 4096 ||
+                            b->max_view_cells > b->total_cells))
+    return bfail(DMGS_EINVAL, "cell_mode 1 needs at most 16 views and 1..4096 cells per view");
   if (b->sh_degree < 0 || b->sh_degree > 3) return bfail(DMGS_EINVAL, "sh_degree must be in [0, 3]");
   if (!b->views || !b->state) return bfail(DMGS_EINVAL, "views and state are required");
   const auto misaligned = [](const void* p) { return p && (reinterpret_cast<uintptr_t>(p) & 255u) != 0; };
@@ -130,8 +135,13 @@ struct Ctx {
       : N((size_t)b->num_views * b->num_gaussians),
         S(N, (size_t)b->total_tiles, (size_t)b->total_pixels,
           dmgs::mv_wave_count_words((int)b->num_gaussians, b->num_views)),
-        L((size_t)b->capacity) {}
+        L((size_t)b->capacity), mode(b->cell_mode),
+        local_bits(b->cell_mode == 1 ? std::max(1, bits_for((uint32_t)(b->max_view_cells - 1))) : 0) {}
   uint32_t* count(const dmgs_batch* b) const { return at<uint32_t>(b->state, S.offsets) + (N - 1); }
+  int mode;
+  int local_bits;
+  // a one-pass sort (view-major lists, at most 256 cells per view) leaves the lists in the b buffers
+  bool lists_in_b() const { return mode == 1 && local_bits <= 8; }
 };
 
 dmgs::MVArgs mv_args(const dmgs_batch* b, const Ctx& c, const float* params) {
@@ -161,8 +171,8 @@ dmgs::RenderArgs render_args(const dmgs_batch* b, const Ctx& c) {
   r.views = reinterpret_cast<const dmgs::ViewDesc*>(b->views);
   r.tile_view = b->tile_view;
   r.ranges = at<uint2>(b->state, c.S.ranges);
-  r.point_list = at<uint32_t>(b->lists, c.L.vals_a);
-  r.list_keys = at<uint32_t>(b->lists, c.L.tkeys_a);
+  r.point_list = at<uint32_t>(b->lists, c.lists_in_b() ? c.L.vals_b : c.L.vals_a);
+  r.list_keys = at<uint32_t>(b->lists, c.lists_in_b() ? c.L.tkeys_b : c.L.tkeys_a);
   r.splats = at<float4>(b->state, c.S.splats);
   r.final_T = at<float>(b->state, c.S.final_T);
   r.n_contrib = at<uint32_t>(b->state, c.S.n_contrib);
@@ -175,9 +185,10 @@ dmgs::RenderArgs render_args(const dmgs_batch* b, const Ctx& c) {
 extern "C" {
 
 int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles, int32_t* total_cells,
-                               int64_t* total_pixels) {
-  if (!views || B <= 0 || !total_tiles || !total_cells || !total_pixels) return bfail(DMGS_EINVAL, "bad arguments");
-  int64_t tiles = 0, cells = 0, pixels = 0;
+                               int32_t* max_view_cells, int64_t* total_pixels) {
+  if (!views || B <= 0 || !total_tiles || !total_cells || !max_view_cells || !total_pixels)
+    return bfail(DMGS_EINVAL, "bad arguments");
+  int64_t tiles = 0, cells = 0, pixels = 0, max_cells = 0;
   for (int v = 0; v < B; ++v) {
     dmgs_view& d = views[v];
     if (d.width <= 0 || d.height <= 0 || !(d.tanfovx > 0.f) || !(d.tanfovy > 0.f))
@@ -191,12 +202,15 @@ int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles
     d.pixel_base = pixels;
     tiles += (int64_t)d.grid_x * d.grid_y;
     const int E = 1 << DMGS_CELL_SHIFT;
-    cells += (int64_t)((d.grid_x + E - 1) / E) * ((d.grid_y + E - 1) / E);
+    const int64_t vc = (int64_t)((d.grid_x + E - 1) / E) * ((d.grid_y + E - 1) / E);
+    cells += vc;
+    max_cells = std::max(max_cells, vc);
     pixels += (int64_t)d.width * d.height;
     if (tiles > (int64_t)0x7fffffff) return bfail(DMGS_EINVAL, "too many tiles");
   }
   *total_tiles = (int32_t)tiles;
-  *total_cells = (int32_t)cells;
+  *total_cells = (int32_t)std::min<int64_t>(cells, 0x7fffffff);
+  *max_view_cells = (int32_t)std::min<int64_t>(max_cells, 0x7fffffff);
   *total_pixels = pixels;
   return DMGS_OK;
 }
@@ -229,14 +243,14 @@ size_t dmgs_batched_backward_size(int32_t B, int64_t P) {
 }
 
 int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, int64_t capacity,
-                         size_t offsets[8]) {
+                         size_t offsets[9]) {
   if (B <= 0 || P <= 0 || total_tiles <= 0 || total_pixels <= 0 || capacity < 0 || (int64_t)B * P > (int64_t)0x7fffffff)
     return bfail(DMGS_EINVAL, "invalid batch sizes");
   if (!offsets) return bfail(DMGS_EINVAL, "offsets is null");
   const size_t N = (size_t)B * P;
   StateLayout S(N, (size_t)total_tiles, (size_t)total_pixels, dmgs::mv_wave_count_words((int)P, B));
   ListLayout Ls((size_t)capacity);
-  offsets[6] = Ls.tkeys_a;
+  offsets[6] = Ls.tkeys_a;  // one-pass view-major sorts leave the lists at tkeys_b / vals_b (see cell_lists)
   offsets[7] = Ls.vals_a;
   offsets[0] = S.radii;                  // (B*P) int32 radii, view-major
   offsets[1] = S.offsets + 4 * (N - 1);  // uint32 instance count of the batch
@@ -244,6 +258,17 @@ int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t tota
   offsets[3] = S.final_T;
   offsets[4] = S.n_contrib;
   offsets[5] = S.splats;
+  offsets[8] = S.status;
+  return DMGS_OK;
+}
+
+int dmgs_batched_list_offsets(const dmgs_batch* b, size_t* keys, size_t* ids) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!keys || !ids) return bfail(DMGS_EINVAL, "bad arguments");
+  const Ctx c(b);
+  *keys = c.lists_in_b() ? c.L.tkeys_b : c.L.tkeys_a;
+  *ids = c.lists_in_b() ? c.L.vals_b : c.L.vals_a;
   return DMGS_OK;
 }
 
@@ -269,7 +294,9 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     dmgs::launch_inclusive_scan(a.wave_counts, a.wave_counts, nw, gs, st);
     const uint32_t* V_dev = a.wave_counts + (nw - 1);
     uint32_t* keys = at<uint32_t>(b->state, c.S.keys_a);
-    dmgs::launch_mv_compact(a, a.depth_keys, keys, a.order, st);
+    uint32_t* status = at<uint32_t>(b->state, c.S.status);
+    dmgs::launch_fill_u32(status, 1, 0u, st);
+    dmgs::launch_mv_compact(a, a.depth_keys, keys, a.order, b->cell_mode == 1, status, st);
     dmgs::radix_sort_pairs(keys, a.order, at<uint32_t>(b->state, c.S.keys_b), at<uint32_t>(b->state, c.S.order_b),
                            N, 32, gs, st, V_dev);
     dmgs::launch_gather_rects(a.rects, a.order, at<uint2>(b->state, c.S.rects_sorted), offs, N, st, V_dev);
@@ -291,12 +318,23 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     d.K = K;
     d.K_dev = c.count(b);
     d.chunk_first = at<uint32_t>(b->lists, c.L.scratch);
+    d.view_major = b->cell_mode == 1;
     dmgs::launch_duplicate(d, st);
-    dmgs::radix_sort_pairs(d.tile_keys, d.values, at<uint32_t>(b->lists, c.L.tkeys_b),
-                           at<uint32_t>(b->lists, c.L.vals_b), K, bits_for((uint32_t)b->total_cells),
-                           at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev);
-    dmgs::launch_tile_ranges(d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells, st, d.K_dev,
-                             DMGS_CELL_KEY_MASK);
+    uint32_t* kb = at<uint32_t>(b->lists, c.L.tkeys_b);
+    uint32_t* vb = at<uint32_t>(b->lists, c.L.vals_b);
+    if (d.view_major) {  // sort the cell-within-view bits only; the view order is already there
+      const bool in_b = dmgs::radix_sort_pairs_min_passes(d.tile_keys, d.values, kb, vb, K,
+                                                          c.local_bits,
+                                                          at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev);
+      if (in_b != c.lists_in_b()) return bfail(DMGS_EHIP, "internal: sorted list buffer mismatch");
+      dmgs::launch_view_cell_ranges(in_b ? kb : d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells,
+                                    a.views, st, d.K_dev);
+    } else {
+      dmgs::radix_sort_pairs(d.tile_keys, d.values, kb, vb, K, bits_for((uint32_t)b->total_cells),
+                             at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev);
+      dmgs::launch_tile_ranges(d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells, st, d.K_dev,
+                               DMGS_CELL_KEY_MASK);
+    }
     if ((rc = bcheck("batch binning", st))) return rc;
   }
   if (stages & DMGS_STAGE_BLEND) {

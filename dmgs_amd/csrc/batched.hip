@@ -100,9 +100,26 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
 // pairs: the depth sort then runs over the visible entries only, with the
 // same stable tie order as over all entries.  wave_counts holds the
 // inclusive scan of the preprocess's per-wave counts.
+//
+// View-major keys: the top byte of a positive float's bits is its exponent
+// field over two, so a visible depth d > 0.2 (>= 2^-3) has a top byte of at
+// least 0x3E and d < 2^29 keeps it at most 0x4D: the top byte fits 4 bits
+// after subtracting 0x3E and the freed 4 bits carry the view.  Sorting
+// (view << 28 | (top - 0x3E) << 24 | low 24 bits) orders by view, then by
+// depth exactly as the float bits do.  A larger depth is clamped to the
+// last bucket and reported in bit 0 of *status.
+__device__ __forceinline__ uint32_t view_major_key(uint32_t depth_bits, int view, bool& clamped) {
+  const uint32_t top = depth_bits >> 24;
+  uint32_t e = top >= 0x3Eu ? top - 0x3Eu : 0u;
+  clamped = top < 0x3Eu || e > 15u;
+  e = min(e, 15u);
+  return ((uint32_t)view << 28) | (e << 24) | (depth_bits & 0xFFFFFFu);
+}
+
 __global__ __launch_bounds__(256) void mv_compact_kernel(MVArgs a, const uint32_t* __restrict__ depth_keys,
                                                          uint32_t* __restrict__ keys_out,
-                                                         uint32_t* __restrict__ ids_out) {
+                                                         uint32_t* __restrict__ ids_out, int view_major,
+                                                         uint32_t* __restrict__ status) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = g < a.P;
   const uint64_t lt = lanemask_lt();
@@ -113,7 +130,13 @@ __global__ __launch_bounds__(256) void mv_compact_kernel(MVArgs a, const uint32_
     const size_t w = ((size_t)v * gridDim.x + blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (visible) {
       const uint32_t slot = (w == 0 ? 0u : a.wave_counts[w - 1]) + (uint32_t)__popcll(bal & lt);
-      keys_out[slot] = depth_keys[i];
+      uint32_t key = depth_keys[i];
+      if (view_major) {
+        bool clamped;
+        key = view_major_key(key, v, clamped);
+        if (clamped) atomicOr(status, 1u);
+      }
+      keys_out[slot] = key;
       ids_out[slot] = (uint32_t)i;
     }
   }
@@ -225,9 +248,10 @@ void launch_mv_preprocess(const MVArgs& a, hipStream_t s) {
 size_t mv_wave_count_words(int P, int B) { return (size_t)B * ((P + 255) / 256) * 4; }
 
 void launch_mv_compact(const MVArgs& a, const uint32_t* depth_keys, uint32_t* keys_out, uint32_t* ids_out,
-                       hipStream_t s) {
+                       bool view_major, uint32_t* status, hipStream_t s) {
   if (a.P <= 0 || a.B <= 0) return;
-  hipLaunchKernelGGL(mv_compact_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a, depth_keys, keys_out, ids_out);
+  hipLaunchKernelGGL(mv_compact_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a, depth_keys, keys_out, ids_out,
+                     view_major ? 1 : 0, status);
 }
 
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s) {

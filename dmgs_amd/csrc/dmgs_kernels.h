@@ -47,6 +47,10 @@ size_t radix_scratch_words(size_t n);
 // n_dev (optional) holds the actual length on the device; n is then the capacity.
 void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
                       uint32_t* scratch, hipStream_t s, const uint32_t* n_dev = nullptr);
+// The same with as few passes as the bits need (one pass up to 8 bits);
+// returns true when the result is in (k1, v1).
+bool radix_sort_pairs_min_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
+                                 uint32_t* scratch, hipStream_t s, const uint32_t* n_dev = nullptr);
 
 // tiles_touched gathered into depth order, ready for the scan.
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int n, hipStream_t s);
@@ -54,7 +58,8 @@ void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, 
 // Emit (tile, splat) pairs in depth order.  Single view: grid_x/grid_y give
 // the tile grid.  Several views: entry g belongs to view g / per_view, the
 // pairs are (binning cell | tile mask << 16, splat) with cells numbered
-// globally from that view's cell_base.
+// globally from that view's cell_base, or with view_major as
+// (cell within the view | view << 12 | tile mask << 16).
 struct DupArgs {
   int P;                        // depth-sorted entries
   const uint32_t* order;
@@ -69,6 +74,7 @@ struct DupArgs {
   uint32_t K;                   // capacity of tile_keys / values
   const uint32_t* K_dev;        // optional device count (clamped to K)
   uint32_t* chunk_first;        // duplicate_scratch_words(K) words
+  bool view_major;
 };
 size_t duplicate_scratch_words(size_t K);
 void launch_duplicate(const DupArgs& a, hipStream_t s);
@@ -76,6 +82,11 @@ void launch_duplicate(const DupArgs& a, hipStream_t s);
 void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s,
                         const uint32_t* K_dev = nullptr, uint32_t key_mask = 0xFFFFFFFFu);
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+// Ranges of view-major lists sorted on the cell-within-view bits: each
+// (view, cell) run is contiguous; ranges are indexed by global cell id
+// (views[view].cell_base + cell).
+void launch_view_cell_ranges(const uint32_t* keys, size_t K, uint2* ranges, int num_cells, const ViewDesc* views,
+                             hipStream_t s, const uint32_t* K_dev);
 
 // ---- multi-view per-Gaussian stages ----
 // Component-major scene parameters (channels x P).
@@ -102,9 +113,11 @@ struct MVArgs {
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s);
 size_t mv_wave_count_words(int P, int B);
 // After an inclusive scan of wave_counts: visible entries in entry order as
-// (depth key, entry id) pairs; their number is wave_counts[last].
+// (depth key, entry id) pairs; their number is wave_counts[last].  With
+// view_major the keys also order by view (at most 16 views; depths of 2^29
+// and more share one bucket and set bit 0 of *status).
 void launch_mv_compact(const MVArgs& a, const uint32_t* depth_keys, uint32_t* keys_out, uint32_t* ids_out,
-                       hipStream_t s);
+                       bool view_major, uint32_t* status, hipStream_t s);
 // rects_sorted[i] = rects[order[i]], counts[i] = its number of binning cells
 // (0 for i >= *n_dev when given).
 void launch_gather_rects(const uint2* rects, const uint32_t* order, uint2* rects_sorted, uint32_t* counts, size_t n,

@@ -326,10 +326,9 @@ size_t radix_scratch_words(size_t n) {
 
 template <int ITEMS>
 static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
-                         uint32_t* scratch, hipStream_t s, const uint32_t* n_dev) {
+                         uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes) {
   constexpr size_t kTile = (size_t)kSortThreads * ITEMS;
   const uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
-  const int passes = bits <= 16 ? 2 : 4;
   int widths[4];
   int left = bits;
   for (int p = 0; p < passes; ++p) {
@@ -359,15 +358,30 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
   }
 }
 
+static void radix_dispatch(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
+                           uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes) {
+  if (sort_items(n) == 16)
+    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes);
+  else
+    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes);
+}
+
 void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
                       uint32_t* scratch, hipStream_t s, const uint32_t* n_dev) {
   if (n == 0) return;
   if (bits < 0) bits = 0;
   if (bits > 32) bits = 32;
-  if (sort_items(n) == 16)
-    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s, n_dev);
-  else
-    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s, n_dev);
+  radix_dispatch(k0, v0, k1, v1, n, bits, scratch, s, n_dev, bits <= 16 ? 2 : 4);
+}
+
+bool radix_sort_pairs_min_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
+                                 uint32_t* scratch, hipStream_t s, const uint32_t* n_dev) {
+  if (n == 0) return false;
+  if (bits < 0) bits = 0;
+  if (bits > 32) bits = 32;
+  const int passes = bits <= 8 ? 1 : (bits <= 16 ? 2 : 4);
+  radix_dispatch(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes);
+  return (passes & 1) != 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -440,7 +454,7 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
         if (v) {  // emit per binning cell of 4x4 tiles
           rx = (uint32_t)rmin.x | ((uint32_t)rmax.x << 16);
           ry = (uint32_t)rmin.y | ((uint32_t)rmax.y << 16);
-          tb = (uint32_t)v->cell_base;
+          tb = a.view_major ? (g / a.per_view) << 12 : (uint32_t)v->cell_base;
           x0 = rmin.x >> DMGS_CELL_SHIFT;
           y0 = rmin.y >> DMGS_CELL_SHIFT;
           w = max(1, ((rmax.x - 1) >> DMGS_CELL_SHIFT) + 1 - x0);
@@ -514,9 +528,16 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
 
 // Four sorted keys per thread (one 16-byte load), each compared with its
 // predecessor: a key that differs from the previous one closes the previous
-// tile's range and opens its own.
+// tile's range and opens its own.  With views, a masked key is
+// (cell within the view | view << 12) and indexes ranges through the view's
+// cell_base.
+__device__ __forceinline__ uint32_t range_slot(uint32_t t, const ViewDesc* views) {
+  return views ? (uint32_t)views[t >> 12].cell_base + (t & 0xFFFu) : t;
+}
+
 __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, uint2* __restrict__ ranges,
-                                   uint32_t num_tiles, const uint32_t* __restrict__ K_dev, uint32_t key_mask) {
+                                   uint32_t num_tiles, const uint32_t* __restrict__ K_dev, uint32_t key_mask,
+                                   const ViewDesc* __restrict__ views) {
   if (K_dev) K = min(K, (size_t)*K_dev);
   const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (i0 >= K) return;
@@ -536,17 +557,18 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, 
   for (int j = 0; j < 4; ++j) {
     const size_t i = i0 + j;
     if (i >= K) break;
-    const uint32_t t = k[j];
+    const uint32_t t = range_slot(k[j], views);
+    const uint32_t pt = range_slot(prev, views);
     if (t < num_tiles) {  // always true for keys from the duplicate step
       if (i == 0) {
         ranges[t].x = 0;
-      } else if (t != prev && prev < num_tiles) {
-        ranges[prev].y = (uint32_t)i;
+      } else if (k[j] != prev && pt < num_tiles) {
+        ranges[pt].y = (uint32_t)i;
         ranges[t].x = (uint32_t)i;
       }
       if (i == K - 1) ranges[t].y = (uint32_t)K;
     }
-    prev = t;
+    prev = k[j];
   }
 }
 
@@ -555,7 +577,15 @@ void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int 
   launch_fill_u32((uint32_t*)ranges, (size_t)num_tiles * 2, 0u, s);
   const size_t threads = (K + 3) / 4;
   if (K) hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, tile_keys,
-                            K, ranges, (uint32_t)num_tiles, K_dev, key_mask);
+                            K, ranges, (uint32_t)num_tiles, K_dev, key_mask, (const ViewDesc*)nullptr);
+}
+
+void launch_view_cell_ranges(const uint32_t* keys, size_t K, uint2* ranges, int num_cells, const ViewDesc* views,
+                             hipStream_t s, const uint32_t* K_dev) {
+  launch_fill_u32((uint32_t*)ranges, (size_t)num_cells * 2, 0u, s);
+  const size_t threads = (K + 3) / 4;
+  if (K) hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, keys, K,
+                            ranges, (uint32_t)num_cells, K_dev, (uint32_t)DMGS_CELL_KEY_MASK, views);
 }
 
 }  // namespace dmgs

@@ -74,7 +74,7 @@ class _Batch:
                                         sh_degree=sh, device=server.device, variant=server.variant)
             self.r._alloc_state()
             self.images = self.r.new_images()
-            self.count = torch.empty(1, dtype=torch.int32, pin_memory=True)
+            self.count = torch.empty(2, dtype=torch.int32, pin_memory=True)  # instances, status word
         self.reserved = 0
         self.K = None
         self.phase = "geometry"
@@ -89,9 +89,14 @@ class _Batch:
             self.event.record(self.stream)
 
     def launch_render(self):
+        stages = BT.STAGE_BINNING | BT.STAGE_BLEND
+        if self.r.cell_mode == 1 and int(self.count[1]) & 1:
+            # a depth the view-major keys cannot order: redo the geometry with global cell lists
+            self.r.cell_mode = 0
+            stages = BT.STAGES_ALL
         with torch.cuda.stream(self.stream):
             self.r.plan(self.params, K_cap=max(self.K, 1), backward=False)
-            self.r.forward(self.params, self.images, stages=BT.STAGE_BINNING | BT.STAGE_BLEND)
+            self.r.forward(self.params, self.images, stages=stages)
             self.event.record(self.stream)
         self.phase = "render"
 

@@ -256,7 +256,7 @@ typedef struct dmgs_view {
  * filtered by its bit, i.e. exactly the reference per-tile list.  At most
  * 65536 cells per batch. */
 int dmgs_batched_prepare_views(dmgs_view* views, int32_t B, int32_t* total_tiles, int32_t* total_cells,
-                               int64_t* total_pixels);
+                               int32_t* max_view_cells, int64_t* total_pixels);
 /* Host-side: tile_view[t] = view owning global tile t (host array of total_tiles). */
 int dmgs_batched_tile_view(const dmgs_view* views, int32_t B, int32_t* tile_view, int32_t total_tiles);
 
@@ -276,10 +276,11 @@ size_t dmgs_batched_backward_size(int32_t B, int64_t P);
  * count of the last forward (uint32), [2] cell ranges (uint2 per global
  * cell), [3] final_T, [4] last contributor (uint32 per packed pixel: one past
  * the position of its last contributing instance in its cell's list),
- * [5] splat records; inside `lists`: [6] sorted keys (cell id | tile mask
- * << 16), [7] the entry ids beside them. */
+ * [5] splat records, [8] status word (uint32; bit 0: a visible depth was too
+ * large for cell_mode 1, see dmgs_batch); inside `lists`: [6] sorted keys
+ * (cell | tile mask << 16), [7] the entry ids beside them. */
 int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, int64_t capacity,
-                         size_t offsets[8]);
+                         size_t offsets[9]);
 
 typedef struct dmgs_batch {
   const dmgs_view* views;   /* device copy of the B prepared records */
@@ -288,7 +289,7 @@ typedef struct dmgs_batch {
   int32_t sh_degree;        /* active SH degree (coefficients are always 16 per colour) */
   int64_t num_gaussians;    /* P */
   float scale_modifier;
-  int32_t variant;          /* blend work-group shape 0..3 (same results; picked by an autotuner) */
+  int32_t variant;          /* blend kernel shape 0..6 (same results; picked by an autotuner) */
   int32_t total_tiles;
   int32_t total_cells;
   int64_t total_pixels;
@@ -296,7 +297,19 @@ typedef struct dmgs_batch {
   void* state;
   void* lists;
   void* bwd_ws;
+  /* List order.  0: instances keyed by global cell id (two sort passes up to
+   * 65536 cells).  1: view-major — the depth sort also orders by view and
+   * instances are keyed by (view, cell within the view), one sort pass when
+   * every view has at most 256 cells; needs num_views <= 16, max_view_cells
+   * <= 4096 and visible depths below 2^29 (a larger one sets bit 0 of the
+   * status word; its order among such far splats is then not guaranteed:
+   * re-run with cell_mode 0). */
+  int32_t cell_mode;
+  int32_t max_view_cells; /* largest cell count of one view (dmgs_batched_prepare_views) */
 } dmgs_batch;
+/* Byte offsets inside `lists` of the sorted keys and entry ids that a forward
+ * of this batch leaves (they depend on cell_mode and max_view_cells). */
+int dmgs_batched_list_offsets(const dmgs_batch* batch, size_t* keys, size_t* ids);
 
 /* Stage mask of the batched calls.  Forward: GEOMETRY = preprocess, depth
  * sort and the scan of tile counts (afterwards the instance count is at

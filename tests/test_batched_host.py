@@ -29,10 +29,10 @@ def test_prepare_views_and_tile_map():
     lib = _lib()
     sizes = [(800, 800), (1920, 1080), (33, 17), (16, 16)]
     cams, views = _views(sizes)
-    tt, tc, tp = N.c_int32(0), N.c_int32(0), N.c_int64(0)
-    N.check(lib.dmgs_batched_prepare_views(views, len(sizes), ctypes.byref(tt), ctypes.byref(tc), ctypes.byref(tp)),
-            "prepare")
-    tiles = cells = pixels = 0
+    tt, tc, mc, tp = N.c_int32(0), N.c_int32(0), N.c_int32(0), N.c_int64(0)
+    N.check(lib.dmgs_batched_prepare_views(views, len(sizes), ctypes.byref(tt), ctypes.byref(tc), ctypes.byref(mc),
+                                           ctypes.byref(tp)), "prepare")
+    tiles = cells = pixels = max_cells = 0
     for v, (w, h) in enumerate(sizes):
         d = views[v]
         assert (d.grid_x, d.grid_y) == ((w + 15) // 16, (h + 15) // 16)
@@ -40,9 +40,11 @@ def test_prepare_views_and_tile_map():
         assert d.focal_x == pytest.approx(w / (2 * math.tan(cams[v].FoVx / 2)), rel=1e-6)
         assert d.focal_y == pytest.approx(h / (2 * math.tan(cams[v].FoVy / 2)), rel=1e-6)
         tiles += d.grid_x * d.grid_y
-        cells += ((w + 63) // 64) * ((h + 63) // 64)  # cells of 4x4 tiles = 64x64 pixels
+        vc = ((w + 63) // 64) * ((h + 63) // 64)  # cells of 4x4 tiles = 64x64 pixels
+        cells += vc
+        max_cells = max(max_cells, vc)
         pixels += w * h
-    assert (tt.value, tc.value, tp.value) == (tiles, cells, pixels)
+    assert (tt.value, tc.value, mc.value, tp.value) == (tiles, cells, max_cells, pixels)
     tv = torch.full((tiles,), -1, dtype=torch.int32)
     N.check(lib.dmgs_batched_tile_view(views, len(sizes), ctypes.c_void_p(tv.data_ptr()), tiles), "tile_view")
     expect = torch.cat([torch.full((views[v].grid_x * views[v].grid_y,), v, dtype=torch.int32)
@@ -56,8 +58,9 @@ def test_prepare_rejects_bad_views():
     lib = _lib()
     _, views = _views([(64, 64), (32, 32)])
     views[1].width = 0
-    tt, tc, tp = N.c_int32(0), N.c_int32(0), N.c_int64(0)
-    assert lib.dmgs_batched_prepare_views(views, 2, ctypes.byref(tt), ctypes.byref(tc), ctypes.byref(tp)) == -1
+    tt, tc, mc, tp = N.c_int32(0), N.c_int32(0), N.c_int32(0), N.c_int64(0)
+    assert lib.dmgs_batched_prepare_views(views, 2, ctypes.byref(tt), ctypes.byref(tc), ctypes.byref(mc),
+                                          ctypes.byref(tp)) == -1
     assert "view 1" in N.last_error()
 
 
@@ -71,10 +74,11 @@ def test_workspace_and_offsets():
     assert lib.dmgs_batched_backward_size(B, P) == B * P * 48
     assert lib.dmgs_batched_state_size(0, P, T, X) == 0
     assert lib.dmgs_batched_lists_size(0) == 0
-    offs = (N.c_size_t * 8)()
+    offs = (N.c_size_t * 9)()
     N.check(lib.dmgs_batched_offsets(B, P, T, X, K, offs), "offsets")
-    assert all(o % 4 == 0 and o < n for o in offs[:6])
-    assert len(set(offs[:6])) == 6
+    state_offs = list(offs[:6]) + [offs[8]]
+    assert all(o % 4 == 0 and o < n for o in state_offs)
+    assert len(set(state_offs)) == 7
     assert offs[6] < offs[7] < lib.dmgs_batched_lists_size(K)
     assert lib.dmgs_batched_offsets(B, 0, T, X, K, offs) == -1
     assert lib.dmgs_batched_offsets(B, 1 << 30, T, X, K, offs) == -1  # more than 2^31 - 1 entries
@@ -86,15 +90,17 @@ def _desc(**kw):
     d.num_views, d.sh_degree, d.num_gaussians = 2, 3, 100
     d.scale_modifier, d.variant, d.total_tiles, d.total_pixels, d.capacity = 1.0, 0, 4, 100, 10
     d.total_cells = 1
+    d.cell_mode, d.max_view_cells = 0, 1
     for k, v in kw.items():
         setattr(d, k, v)
     return ctypes.byref(d)
 
 
 def test_batch_descriptor_layout():
-    # struct dmgs_batch: 2 pointers, 2 int32, int64, float, 3 int32, 2 int64, 3 pointers
-    assert ctypes.sizeof(N.DmgsBatch) == 88
+    # struct dmgs_batch: 2 pointers, 2 int32, int64, float, 3 int32, 2 int64, 3 pointers, 2 int32
+    assert ctypes.sizeof(N.DmgsBatch) == 96
     assert N.DmgsBatch.capacity.offset == 56 and N.DmgsBatch.state.offset == 64
+    assert N.DmgsBatch.cell_mode.offset == 88 and N.DmgsBatch.max_view_cells.offset == 92
     assert N.DmgsBatch.total_cells.offset == 44 and N.DmgsView.cell_base.offset == 196
     assert ctypes.sizeof(N.DmgsView) == 208
 
@@ -119,6 +125,21 @@ def test_entry_points_validate_before_launch():
     assert "aligned" in N.last_error()
     assert lib.dmgs_batched_forward(_desc(total_cells=0), fake, fake, A, None) == -1
     assert lib.dmgs_batched_forward(_desc(total_cells=5), fake, fake, A, None) == -1  # more cells than tiles
+    # view-major lists: at most 16 views of at most 4096 cells
+    assert lib.dmgs_batched_forward(_desc(cell_mode=2), fake, fake, A, None) == -1
+    assert "cell_mode" in N.last_error()
+    assert lib.dmgs_batched_forward(_desc(cell_mode=1, num_views=17), fake, fake, A, None) == -1
+    assert lib.dmgs_batched_forward(_desc(cell_mode=1, max_view_cells=0), fake, fake, A, None) == -1
+    assert lib.dmgs_batched_forward(_desc(cell_mode=1, total_tiles=1 << 16, total_cells=8192, max_view_cells=4097),
+                                    fake, fake, A, None) == -1
+    # sorted lists: in the first buffers, but in the second after a one-pass (<= 256 cells per view) sort
+    ko, vo = N.c_size_t(0), N.c_size_t(0)
+    offs = (N.c_size_t * 9)()
+    N.check(lib.dmgs_batched_offsets(2, 100, 4, 100, 10, offs), "offsets")
+    N.check(lib.dmgs_batched_list_offsets(_desc(), ctypes.byref(ko), ctypes.byref(vo)), "list offsets")
+    assert (ko.value, vo.value) == (offs[6], offs[7])
+    N.check(lib.dmgs_batched_list_offsets(_desc(cell_mode=1), ctypes.byref(ko), ctypes.byref(vo)), "list offsets")
+    assert len({ko.value, vo.value, offs[6], offs[7]}) == 4
     for bad in (0, 8, -1):
         assert lib.dmgs_batched_forward(_desc(), fake, fake, bad, None) == -1
         assert "stage mask" in N.last_error()

@@ -12,6 +12,8 @@
 * A captured HIP graph of the whole step reproduces the eager step on every
   replay, with eager work and synchronisation between replays.
 """
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -81,9 +83,9 @@ def _cpu(cam):
     return c
 
 
-def _batched(model_cpu, cams, bgs, variant=0):
+def _batched(model_cpu, cams, bgs, variant=0, cell_mode=None):
     r = BT.BatchedRenderer(cams, [b.cuda() for b in bgs], model_cpu.num, sh_degree=model_cpu.max_sh_degree,
-                           variant=variant)
+                           variant=variant, cell_mode=cell_mode)
     params = BT.pack_model(model_cpu.to("cuda"))
     r.plan(params)
     return r, params
@@ -260,21 +262,39 @@ def test_graph_replay_matches_eager():
     assert not r.overflowed()
 
 
-@pytest.mark.parametrize("sizes", [[(200, 150), (77, 53), (256, 256)], [(33, 17)]])
-def test_cell_lists_equal_reference_tile_lists(sizes):
+@pytest.mark.parametrize("cell_mode,sizes", [(0, [(200, 150), (77, 53), (256, 256)]), (0, [(33, 17)]),
+                                              (1, [(200, 150), (77, 53), (256, 256)]), (1, [(33, 17)]),
+                                              (1, [(1100, 1000), (64, 64)])])
+def test_cell_lists_equal_reference_tile_lists(cell_mode, sizes):
     """Every tile's list, read from its binning cell's list filtered by the
     tile's tag bit, equals the reference (tile | depth)-sorted list of that
     tile (oracle binning of the same FP32 splat positions, radii and
-    depths): bit-exact entry ids in the same order."""
-    print(f"case: cell lists vs reference tile lists, sizes={sizes}", flush=True)
+    depths): bit-exact entry ids in the same order.  Cell mode 0 keys lists
+    by global cell id; mode 1 by (view, cell within the view) with one sort
+    pass for views of up to 256 cells and two beyond (the 1100x1000 view has
+    323 cells)."""
+    print(f"case: cell lists vs reference tile lists, cell_mode={cell_mode} sizes={sizes}", flush=True)
     model = S.free_scene(20000, seed=6, layout="mixture")
     cams = _cameras(sizes, seed=19)
-    r, params = _batched(model, cams, _bgs(len(cams)))
+    r, params = _batched(model, cams, _bgs(len(cams)), cell_mode=cell_mode)
+    assert r.cell_mode == cell_mode
     r.forward(params)
     torch.cuda.synchronize()
+    assert not r.overflowed()
     ranges, keys, vals = r.cell_lists()
     ranges, keys, vals = ranges.numpy(), keys.numpy().view(np.uint32), vals.numpy().view(np.uint32)
-    assert np.all(np.diff((keys & 0xFFFF).astype(np.int64)) >= 0), "cell ids must be sorted"
+    if cell_mode == 0:
+        assert np.all(np.diff((keys & 0xFFFF).astype(np.int64)) >= 0), "cell ids must be sorted"
+    else:
+        assert np.all(np.diff((keys & 0xFFF).astype(np.int64)) >= 0), "cells within a view must be sorted"
+        view = (keys >> 12) & 0xF
+        assert np.array_equal(view, vals // r.P), "key view and entry view differ"
+        base = np.array([r.views[v].cell_base for v in range(r.B)], dtype=np.int64)
+        glob = base[view] + (keys & 0xFFF)
+        # each (view, cell) run is contiguous and is exactly its range
+        for c in np.unique(glob):
+            pos = np.nonzero(glob == c)[0]
+            assert pos[-1] - pos[0] + 1 == pos.size and tuple(ranges[c]) == (pos[0], pos[-1] + 1)
     o = r._offsets[5]
     splats = r.state[o:o + 48 * r.B * r.P].view(torch.float32).view(r.B * r.P, 12).cpu().numpy()
     ids = vals
@@ -303,3 +323,31 @@ def test_cell_lists_equal_reference_tile_lists(sizes):
             assert np.array_equal(mine, ref), f"view {v} tile {t}: {mine[:8]} vs {ref[:8]}"
             checked += ref.size
     assert checked > 0
+
+
+def test_view_major_far_depth_falls_back():
+    """A visible splat at depth >= 2^29 cannot be ordered by view-major keys:
+    the forward reports it, the renderer switches to global cell lists, and
+    the re-rendered images equal those of a cell-mode-0 renderer."""
+    print("case: view-major far depth fallback", flush=True)
+    model = S.free_scene(4000, seed=8, layout="uniform")
+    cams = _cameras([(96, 64), (64, 64)], seed=3)
+    # one huge splat far along the first camera's optical axis
+    c = cams[0]
+    Rt = c.world_view_transform.detach().cpu().double()
+    center, fwd = c.camera_center.detach().cpu().double(), Rt[:3, 2]
+    with torch.no_grad():
+        model._xyz[0] = (center + 6e8 * fwd).float()  # 2^29 = 5.4e8
+        model._scaling[0] = math.log(4e7)
+        model._opacity[0] = 5.0
+    bgs = _bgs(len(cams))
+    r, params = _batched(model, cams, bgs, cell_mode=1)
+    img = r.forward(params)
+    torch.cuda.synchronize()
+    assert int(r.radii()[0, 0]) > 0, "the far splat must be visible"
+    assert r.depth_clamped()
+    assert r.overflowed() and r.cell_mode == 0
+    img = r.forward(params)
+    assert not r.overflowed()
+    r0, _ = _batched(model, cams, bgs, cell_mode=0)
+    assert torch.equal(img, r0.forward(params))
