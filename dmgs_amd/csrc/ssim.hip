@@ -7,100 +7,172 @@
 // the mean absolute difference.  The 2-D window is the outer product of the
 // normalised 1-D window, so both passes blur separably through LDS.
 //
-// Forward: one 16x16 output tile per workgroup (channel in blockIdx.z) stages
-// a 26x26 patch of both images, blurs x, y, x^2, y^2, xy, evaluates the SSIM
-// map, writes per-block partial sums (deterministic two-level reduction) and
-// the three per-pixel coefficient maps the backward needs:
+// Forward: one 32x32 output tile per workgroup (plane in blockIdx.z) stages
+// the 42x42 patch of both images in LDS.  The horizontal blur gives each
+// task four neighbouring outputs of one patch row from 14 loaded values
+// (x, y and the three products formed once per value); the vertical blur
+// gives each thread four outputs of one column from 14 blurred rows.  The
+// kernel evaluates the SSIM map, writes per-block partial sums
+// (deterministic two-level reduction) and the three per-pixel coefficient
+// maps the backward needs:
 //   a = dS/dmu1, b = dS/d(E[x^2]), c = dS/d(E[xy])   (each times dL/dS).
-// Backward: dL/dx = G*a + 2 x (G*b) + y (G*c) + (1 - lambda) sign(x - y) / N.
+// Backward: dL/dx = G*a + 2 x (G*b) + y (G*c) + (1 - lambda) sign(x - y) / N,
+// with the same tiling over the three coefficient planes.
 #include "dmgs_common.h"
 #include "dmgs_kernels.h"
 
 namespace dmgs {
 
-constexpr int kT = 16;            // output tile edge
-constexpr int kR = 5;             // window radius
-constexpr int kP = kT + 2 * kR;   // 26, staged patch edge
+constexpr int kTW = 32, kTH = 32;              // output tile
+constexpr int kR = 5;                          // window radius
+constexpr int kPW = kTW + 2 * kR;              // 42 staged columns
+constexpr int kPH = kTH + 2 * kR;              // 42 staged rows
+constexpr int kPS = 44;                        // staged row stride (16-byte rows)
+constexpr int kHS = 40;                        // blurred row stride
+constexpr int kQuads = kTW / 4;                // horizontal tasks per row
+constexpr int kHTasks = kPH * kQuads;          // 336
+static_assert(kTW == 32 && kTH == 32, "the vertical pass maps 256 threads to 32 columns x 8 row quads");
 
 struct Window11 {
   float w[11];
 };
+
+// Stage zero-padded kPH x kPW patches of NP planes (origin at (oy - 5, ox - 5)).
+// Every global load of the thread is issued before the first LDS store, so
+// the workgroup waits for one memory latency, not one per element.
+constexpr int kStageIters = (kPH * kPW + 255) / 256;  // 7
+template <int NP>
+__device__ __forceinline__ void stage_patches(float (*const dst[NP])[kPS], const float* const src[NP], int H, int W,
+                                              int ox, int oy) {
+  float v[NP][kStageIters];
+#pragma unroll
+  for (int it = 0; it < kStageIters; ++it) {
+    const int i = threadIdx.x + it * 256;
+    const int r = i / kPW, q = i - r * kPW;
+    const int gy = oy + r - kR, gx = ox + q - kR;
+    const bool in = i < kPH * kPW && gy >= 0 && gy < H && gx >= 0 && gx < W;
+    const size_t o = in ? (size_t)gy * W + gx : 0;
+#pragma unroll
+    for (int m = 0; m < NP; ++m) v[m][it] = in ? src[m][o] : 0.f;
+  }
+#pragma unroll
+  for (int it = 0; it < kStageIters; ++it) {
+    const int i = threadIdx.x + it * 256;
+    if (i >= kPH * kPW) break;
+    const int r = i / kPW, q = i - r * kPW;
+#pragma unroll
+    for (int m = 0; m < NP; ++m) dst[m][r][q] = v[m][it];
+  }
+}
+
+// The 14 staged values of row r starting at column 4j (16-byte aligned).
+__device__ __forceinline__ void load14(const float (*src)[kPS], int r, int j, float v[14]) {
+  const float4* p = reinterpret_cast<const float4*>(&src[r][4 * j]);
+  const float4 a = p[0], b = p[1], c = p[2];
+  const float2 d = *reinterpret_cast<const float2*>(&src[r][4 * j + 12]);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  v[8] = c.x; v[9] = c.y; v[10] = c.z; v[11] = c.w;
+  v[12] = d.x; v[13] = d.y;
+}
 
 // One loss per group of `group` consecutive planes (group = C: one loss over
 // the whole image; group = 3: one per RGB view of a batch).
 __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                         int H, int W, int group, Window11 g, float lam,
                                                         float* __restrict__ coef, float* __restrict__ partials) {
-  __shared__ float sx[kP][kP + 1], sy[kP][kP + 1];
-  __shared__ float hq[5][kP][kT + 1];
+  __shared__ __align__(16) float sx[kPH][kPS];
+  __shared__ __align__(16) float sy[kPH][kPS];
+  __shared__ __align__(16) float hq[5][kPH][kHS];
   __shared__ float red[2][4];
   const int c = blockIdx.z;
-  const int ox = blockIdx.x * kT, oy = blockIdx.y * kT;
+  const int ox = blockIdx.x * kTW, oy = blockIdx.y * kTH;
   const size_t plane = (size_t)H * W;
-  const float* xc = x + c * plane;
-  const float* yc = y + c * plane;
-  for (int i = threadIdx.x; i < kP * kP; i += 256) {
-    const int r = i / kP, q = i % kP;
-    const int gy = oy + r - kR, gx = ox + q - kR;
-    const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
-    sx[r][q] = in ? xc[(size_t)gy * W + gx] : 0.f;
-    sy[r][q] = in ? yc[(size_t)gy * W + gx] : 0.f;
+  {
+    float(*const dst[2])[kPS] = {sx, sy};
+    const float* const src[2] = {x + c * plane, y + c * plane};
+    stage_patches<2>(dst, src, H, W, ox, oy);
   }
   __syncthreads();
-  // horizontal pass over all 26 rows for the 16 output columns
-  for (int i = threadIdx.x; i < kP * kT; i += 256) {
-    const int r = i / kT, q = i % kT;
-    float m1 = 0.f, m2 = 0.f, s11 = 0.f, s22 = 0.f, s12 = 0.f;
+  // horizontal: four outputs of one staged row per task
+  for (int t = threadIdx.x; t < kHTasks; t += 256) {
+    const int r = t / kQuads, j = t - r * kQuads;
+    float a[14], b[14];
+    load14(sx, r, j, a);
+    load14(sy, r, j, b);
+    float m1[4] = {0.f, 0.f, 0.f, 0.f}, m2[4] = {0.f, 0.f, 0.f, 0.f}, s11[4] = {0.f, 0.f, 0.f, 0.f},
+          s22[4] = {0.f, 0.f, 0.f, 0.f}, s12[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k = 0; k < 11; ++k) {
-      const float a = sx[r][q + k], b = sy[r][q + k], w = g.w[k];
-      m1 += w * a;
-      m2 += w * b;
-      s11 += w * (a * a);
-      s22 += w * (b * b);
-      s12 += w * (a * b);
+    for (int e = 0; e < 14; ++e) {
+      const float aa = a[e] * a[e], bb = b[e] * b[e], ab = a[e] * b[e];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = e - i;
+        if (k < 0 || k > 10) continue;
+        const float w = g.w[k];
+        m1[i] = fmaf(w, a[e], m1[i]);
+        m2[i] = fmaf(w, b[e], m2[i]);
+        s11[i] = fmaf(w, aa, s11[i]);
+        s22[i] = fmaf(w, bb, s22[i]);
+        s12[i] = fmaf(w, ab, s12[i]);
+      }
     }
-    hq[0][r][q] = m1;
-    hq[1][r][q] = m2;
-    hq[2][r][q] = s11;
-    hq[3][r][q] = s22;
-    hq[4][r][q] = s12;
+    *reinterpret_cast<float4*>(&hq[0][r][4 * j]) = make_float4(m1[0], m1[1], m1[2], m1[3]);
+    *reinterpret_cast<float4*>(&hq[1][r][4 * j]) = make_float4(m2[0], m2[1], m2[2], m2[3]);
+    *reinterpret_cast<float4*>(&hq[2][r][4 * j]) = make_float4(s11[0], s11[1], s11[2], s11[3]);
+    *reinterpret_cast<float4*>(&hq[3][r][4 * j]) = make_float4(s22[0], s22[1], s22[2], s22[3]);
+    *reinterpret_cast<float4*>(&hq[4][r][4 * j]) = make_float4(s12[0], s12[1], s12[2], s12[3]);
   }
   __syncthreads();
-  const int lx = threadIdx.x % kT, ly = threadIdx.x / kT;
-  const int px = ox + lx, py = oy + ly;
-  float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  // vertical: four outputs of one column per thread
+  const int lx = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
+  float v[4][5];
 #pragma unroll
-  for (int k = 0; k < 11; ++k) {
-    const float w = g.w[k];
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int q = 0; q < 5; ++q) v[q] += w * hq[q][ly + k][lx];
+    for (int q = 0; q < 5; ++q) v[i][q] = 0.f;
+#pragma unroll
+  for (int e = 0; e < 14; ++e) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const float h = hq[q][r0 + e][lx];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = e - i;
+        if (k < 0 || k > 10) continue;
+        v[i][q] = fmaf(g.w[k], h, v[i][q]);
+      }
+    }
   }
   const float C1 = 0.01f * 0.01f, C2 = 0.03f * 0.03f;
-  const float mu1 = v[0], mu2 = v[1];
-  const float mu1_sq = mu1 * mu1, mu2_sq = mu2 * mu2, mu12 = mu1 * mu2;
-  const float sig1 = v[2] - mu1_sq, sig2 = v[3] - mu2_sq, sig12 = v[4] - mu12;
-  const float A1 = 2.f * mu12 + C1, A2 = 2.f * sig12 + C2;
-  const float B1 = mu1_sq + mu2_sq + C1, B2 = sig1 + sig2 + C2;
-  const bool inside = px < W && py < H;
-  float S = 0.f, l1 = 0.f;
-  if (inside) {
-    S = (A1 * A2) / (B1 * B2);
-    l1 = fabsf(sx[ly + kR][lx + kR] - sy[ly + kR][lx + kR]);
-    // dL/dS for the mean over the group's group*H*W pixels
-    const float dS = -lam / (float)((size_t)group * plane);
+  const float dS = -lam / (float)((size_t)group * plane);  // dL/dS for the mean over the group's pixels
+  const size_t all = (size_t)gridDim.z * plane;
+  const int px = ox + lx;
+  float s_acc = 0.f, l_acc = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int py = oy + r0 + i;
+    if (px >= W || py >= H) continue;
+    const float mu1 = v[i][0], mu2 = v[i][1];
+    const float mu1_sq = mu1 * mu1, mu2_sq = mu2 * mu2, mu12 = mu1 * mu2;
+    const float sig1 = v[i][2] - mu1_sq, sig2 = v[i][3] - mu2_sq, sig12 = v[i][4] - mu12;
+    const float A1 = 2.f * mu12 + C1, A2 = 2.f * sig12 + C2;
+    const float B1 = mu1_sq + mu2_sq + C1, B2 = sig1 + sig2 + C2;
     const float inv = 1.f / (B1 * B2);
-    const float dmu1 = (2.f * mu2 * A2 - 2.f * mu2 * A1) * inv - S * (2.f * mu1 / B1 - 2.f * mu1 / B2);
-    const float ds11 = -S / B2;
+    const float S = (A1 * A2) * inv;
+    s_acc += S;
+    l_acc += fabsf(sx[r0 + i + kR][lx + kR] - sy[r0 + i + kR][lx + kR]);
+    // 1/B1 = B2 inv and 1/B2 = B1 inv
+    const float dmu1 = inv * (2.f * mu2 * (A2 - A1) - S * (2.f * mu1) * (B2 - B1));
+    const float ds11 = -S * B1 * inv;
     const float ds12 = 2.f * A1 * inv;
     const size_t o = c * plane + (size_t)py * W + px;
-    const size_t all = (size_t)gridDim.z * plane;
     coef[o] = dS * dmu1;
     coef[all + o] = dS * ds11;
     coef[2 * all + o] = dS * ds12;
   }
   // block sums of S and |x - y|
-  const float s_sum = wave_sum(S), l_sum = wave_sum(l1);
+  const float s_sum = wave_sum(s_acc), l_sum = wave_sum(l_acc);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) {
     red[0][wave] = s_sum;
@@ -148,60 +220,83 @@ __global__ __launch_bounds__(256) void dssim_bwd_kernel(const float* __restrict_
                                                         const float* __restrict__ coef, int H, int W, int C,
                                                         int group, Window11 g, float lam,
                                                         const float* __restrict__ dloss, float* __restrict__ dx) {
-  __shared__ float sa[3][kP][kP + 1];
-  __shared__ float hq[3][kP][kT + 1];
+  __shared__ __align__(16) float sa[3][kPH][kPS];
+  __shared__ __align__(16) float hq[3][kPH][kHS];
   const int c = blockIdx.z;
-  const int ox = blockIdx.x * kT, oy = blockIdx.y * kT;
+  const int ox = blockIdx.x * kTW, oy = blockIdx.y * kTH;
   const size_t plane = (size_t)H * W;
   const size_t all = (size_t)C * plane;
-  for (int i = threadIdx.x; i < kP * kP; i += 256) {
-    const int r = i / kP, q = i % kP;
-    const int gy = oy + r - kR, gx = ox + q - kR;
-    const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
-    const size_t o = c * plane + (size_t)gy * W + gx;
+  const int lx = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
+  const int px = ox + lx;
+  // the thread's own pixels of x and y, loaded while the patches stage
+  float xv[4], yv[4];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) sa[m][r][q] = in ? coef[m * all + o] : 0.f;
+  for (int i = 0; i < 4; ++i) {
+    const int py = oy + r0 + i;
+    const bool in = px < W && py < H;
+    const size_t o = c * plane + (in ? (size_t)py * W + px : 0);
+    xv[i] = in ? x[o] : 0.f;
+    yv[i] = in ? y[o] : 0.f;
+  }
+  {
+    float(*const dst[3])[kPS] = {sa[0], sa[1], sa[2]};
+    const float* const src[3] = {coef + c * plane, coef + all + c * plane, coef + 2 * all + c * plane};
+    stage_patches<3>(dst, src, H, W, ox, oy);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kP * kT; i += 256) {
-    const int r = i / kT, q = i % kT;
-    float h0 = 0.f, h1 = 0.f, h2 = 0.f;
+  for (int t = threadIdx.x; t < kHTasks; t += 256) {
+    const int r = t / kQuads, j = t - r * kQuads;
 #pragma unroll
-    for (int k = 0; k < 11; ++k) {
-      const float w = g.w[k];
-      h0 += w * sa[0][r][q + k];
-      h1 += w * sa[1][r][q + k];
-      h2 += w * sa[2][r][q + k];
+    for (int m = 0; m < 3; ++m) {
+      float a[14];
+      load14(sa[m], r, j, a);
+      float h[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int e = 0; e < 14; ++e)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = e - i;
+          if (k < 0 || k > 10) continue;
+          h[i] = fmaf(g.w[k], a[e], h[i]);
+        }
+      *reinterpret_cast<float4*>(&hq[m][r][4 * j]) = make_float4(h[0], h[1], h[2], h[3]);
     }
-    hq[0][r][q] = h0;
-    hq[1][r][q] = h1;
-    hq[2][r][q] = h2;
   }
   __syncthreads();
-  const int lx = threadIdx.x % kT, ly = threadIdx.x / kT;
-  const int px = ox + lx, py = oy + ly;
-  if (px >= W || py >= H) return;
-  float b0 = 0.f, b1 = 0.f, b2 = 0.f;
+  float b[4][3];
 #pragma unroll
-  for (int k = 0; k < 11; ++k) {
-    const float w = g.w[k];
-    b0 += w * hq[0][ly + k][lx];
-    b1 += w * hq[1][ly + k][lx];
-    b2 += w * hq[2][ly + k][lx];
+  for (int i = 0; i < 4; ++i) b[i][0] = b[i][1] = b[i][2] = 0.f;
+#pragma unroll
+  for (int e = 0; e < 14; ++e)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const float h = hq[m][r0 + e][lx];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = e - i;
+        if (k < 0 || k > 10) continue;
+        b[i][m] = fmaf(g.w[k], h, b[i][m]);
+      }
+    }
+  if (px >= W) return;
+  const float scale = dloss[c / group];
+  const float g_l1 = (1.f - lam) / (float)((size_t)group * plane);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int py = oy + r0 + i;
+    if (py >= H) break;
+    const size_t o = c * plane + (size_t)py * W + px;
+    const float d = xv[i] - yv[i];
+    const float sgn = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
+    dx[o] = scale * (b[i][0] + 2.f * xv[i] * b[i][1] + yv[i] * b[i][2] + g_l1 * sgn);
   }
-  const size_t o = c * plane + (size_t)py * W + px;
-  const float xv = x[o], yv = y[o];
-  const float d = xv - yv;
-  const float sgn = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
-  const float g_l1 = (1.f - lam) * sgn / (float)((size_t)group * plane);
-  dx[o] = dloss[c / group] * (b0 + 2.f * xv * b1 + yv * b2 + g_l1);
 }
 
 void launch_dssim_forward(const float* x, const float* y, int C, int group, int H, int W, const float* w11,
                           float lam, float* coef, float* partials, float* out, hipStream_t s) {
   Window11 g;
   for (int k = 0; k < 11; ++k) g.w[k] = w11[k];
-  const dim3 grid((W + kT - 1) / kT, (H + kT - 1) / kT, C);
+  const dim3 grid((W + kTW - 1) / kTW, (H + kTH - 1) / kTH, C);
   hipLaunchKernelGGL(dssim_fwd_kernel, grid, dim3(256), 0, s, x, y, H, W, group, g, lam, coef, partials);
   const int per_group = (int)(grid.x * grid.y) * group;
   hipLaunchKernelGGL(dssim_reduce_kernel, dim3(C / group), dim3(256), 0, s, (const float*)partials, per_group,
@@ -212,12 +307,12 @@ void launch_dssim_backward(const float* x, const float* y, const float* coef, in
                            const float* w11, float lam, const float* dloss, float* dx, hipStream_t s) {
   Window11 g;
   for (int k = 0; k < 11; ++k) g.w[k] = w11[k];
-  const dim3 grid((W + kT - 1) / kT, (H + kT - 1) / kT, C);
+  const dim3 grid((W + kTW - 1) / kTW, (H + kTH - 1) / kTH, C);
   hipLaunchKernelGGL(dssim_bwd_kernel, grid, dim3(256), 0, s, x, y, coef, H, W, C, group, g, lam, dloss, dx);
 }
 
 size_t dssim_partials(int C, int H, int W) {
-  return (size_t)2 * ((W + kT - 1) / kT) * ((H + kT - 1) / kT) * C;
+  return (size_t)2 * ((W + kTW - 1) / kTW) * ((H + kTH - 1) / kTH) * C;
 }
 
 }  // namespace dmgs
