@@ -70,8 +70,7 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
   MVGaussian G;
   if (live) load_mv(a, g, G);
   for (int v = 0; v < a.B; ++v) {
-    const ViewDesc& vd = a.views[v];
-    const ViewParams vp = view_params(vd);
+    const ViewParams vp = view_params(const_ptr(a.views)[__builtin_amdgcn_readfirstlane(v)]);
     const size_t i = (size_t)v * a.P + g;
     SplatGeom o;
     const bool visible = live && project_splat(G.p, G.c3, vp, o);
@@ -81,7 +80,7 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
       a.radii[i] = 0;
       continue;
     }
-    float3 dir = make_float3(G.p.x - vd.campos[0], G.p.y - vd.campos[1], G.p.z - vd.campos[2]);
+    float3 dir = make_float3(G.p.x - vp.campos[0], G.p.y - vp.campos[1], G.p.z - vp.campos[2]);
     const float inv_len = 1.0f / sqrtf(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
     dir = make_float3(dir.x * inv_len, dir.y * inv_len, dir.z * inv_len);
     float3 rgb = sh_to_rgb(a.sh_degree, G.sh, dir);
@@ -158,12 +157,11 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
     const size_t i = (size_t)v * a.P + g;
     if (a.radii[i] <= 0) continue;
     any = true;
-    const ViewDesc& vd = a.views[v];
-    const ViewParams vp = view_params(vd);
     float* gr = a.gacc + i * DMGS_GACC_STRIDE;
     const float4 g03 = *reinterpret_cast<const float4*>(gr);
     const float4 g47 = *reinterpret_cast<const float4*>(gr + 4);
     const float g8 = gr[8];
+    const ViewParams vp = view_params(const_ptr(a.views)[__builtin_amdgcn_readfirstlane(v)]);
     // hand the row back zeroed: the next blend backward accumulates into zeros
     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     reinterpret_cast<float4*>(gr)[0] = z;
@@ -172,7 +170,7 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
     const float3 dm = ewa_backward(G.p, G.c3, vp, g03.x, g03.y, g03.z, g03.w, g47.x, dc);
     float b[16];
     float3 gm;
-    const float3 dd = sh_backward(a.sh_degree, G.sh, G.p, vd.campos, make_float3(g47.z, g47.w, g8), b, gm);
+    const float3 dd = sh_backward(a.sh_degree, G.sh, G.p, vp.campos, make_float3(g47.z, g47.w, g8), b, gm);
     dmean.x += dm.x + dd.x;
     dmean.y += dm.y + dd.y;
     dmean.z += dm.z + dd.z;

@@ -85,16 +85,8 @@ __global__ __launch_bounds__(256) void geometry_bwd_kernel(GeometryBwdArgs a) {
     quat_to_R(q, R);
     cov3d_from_sR(s3, R, c);
   }
-  ViewParams v;
-  v.vm = a.viewmatrix;
-  v.pm = a.projmatrix;
-  v.campos = a.campos;
-  v.W = a.W;
-  v.H = a.H;
-  v.fx = a.focal_x;
-  v.fy = a.focal_y;
-  v.tanx = a.tanfovx;
-  v.tany = a.tanfovy;
+  const ViewParams v = view_params(a.viewmatrix, a.projmatrix, a.campos, a.W, a.H, 0, 0, a.focal_x, a.focal_y,
+                                   a.tanfovx, a.tanfovy);
   float dc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   float3 dmean = ewa_backward(p, c, v, dmx, dmy, dqa, dqb, dqc, dc);
   if (a.dL_dcov3D)
@@ -104,7 +96,7 @@ __global__ __launch_bounds__(256) void geometry_bwd_kernel(GeometryBwdArgs a) {
     float sh[48], b[16];
     float3 gm;
     load_sh16(a.shs + (size_t)idx * 3 * a.sh_coeffs, a.sh_coeffs, sh);
-    const float3 dm = sh_backward(a.sh_degree, sh, p, a.campos, make_float3(g47.z, g47.w, g8), b, gm);
+    const float3 dm = sh_backward(a.sh_degree, sh, p, v.campos, make_float3(g47.z, g47.w, g8), b, gm);
     store_dsh(a.dL_dsh + (size_t)idx * 3 * a.sh_coeffs, a.sh_coeffs, b, gm);
     dmean.x += dm.x;
     dmean.y += dm.y;

@@ -78,20 +78,34 @@ __device__ __forceinline__ float3 sh_to_rgb(int deg, const float (&sh)[48], floa
   return r;
 }
 
-// One view of a camera, as the per-Gaussian kernels need it.
+// Read-only kernel inputs seen through the constant address space: loads
+// from a wave-uniform address become scalar loads (one per wave, through the
+// scalar cache) instead of per-lane vector loads.
+#define DMGS_CONST_AS __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const DMGS_CONST_AS T* const_ptr(const T* p) {
+  return (const DMGS_CONST_AS T*)(p);
+}
+
+// One view of a camera, as the per-Gaussian kernels need it (values: with a
+// uniform view they live in scalar registers).
 struct ViewParams {
-  const float* vm;
-  const float* pm;
-  const float* campos;
+  float vm[16];
+  float pm[16];
+  float campos[3];
   int W, H, gx, gy;
   float fx, fy, tanx, tany;
 };
 
-__device__ __forceinline__ ViewParams view_params(const ViewDesc& v) {
+__device__ __forceinline__ ViewParams view_params(const DMGS_CONST_AS ViewDesc& v) {
   ViewParams p;
-  p.vm = v.view;
-  p.pm = v.proj;
-  p.campos = v.campos;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    p.vm[k] = v.view[k];
+    p.pm[k] = v.proj[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) p.campos[k] = v.campos[k];
   p.W = v.W;
   p.H = v.H;
   p.gx = v.grid_x;
@@ -100,6 +114,32 @@ __device__ __forceinline__ ViewParams view_params(const ViewDesc& v) {
   p.fy = v.focal_y;
   p.tanx = v.tanfovx;
   p.tany = v.tanfovy;
+  return p;
+}
+
+// The same from the single-view kernels' separate camera arrays.
+__device__ __forceinline__ ViewParams view_params(const float* viewmatrix, const float* projmatrix,
+                                                  const float* campos, int W, int H, int gx, int gy, float fx,
+                                                  float fy, float tanx, float tany) {
+  ViewParams p;
+  const DMGS_CONST_AS float* vm = const_ptr(viewmatrix);
+  const DMGS_CONST_AS float* pm = const_ptr(projmatrix);
+  const DMGS_CONST_AS float* cp = const_ptr(campos);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    p.vm[k] = vm[k];
+    p.pm[k] = pm[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) p.campos[k] = cp[k];
+  p.W = W;
+  p.H = H;
+  p.gx = gx;
+  p.gy = gy;
+  p.fx = fx;
+  p.fy = fy;
+  p.tanx = tanx;
+  p.tany = tany;
   return p;
 }
 

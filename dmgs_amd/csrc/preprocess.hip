@@ -37,18 +37,8 @@ __global__ __launch_bounds__(256) void preprocess_fwd_kernel(PreprocessArgs a) {
     quat_to_R(reinterpret_cast<const float4*>(a.rotations)[idx], R);
     cov3d_from_sR(s, R, c3);
   }
-  ViewParams v;
-  v.vm = a.viewmatrix;
-  v.pm = a.projmatrix;
-  v.campos = a.campos;
-  v.W = a.W;
-  v.H = a.H;
-  v.gx = a.grid_x;
-  v.gy = a.grid_y;
-  v.fx = a.focal_x;
-  v.fy = a.focal_y;
-  v.tanx = a.tanfovx;
-  v.tany = a.tanfovy;
+  const ViewParams v = view_params(a.viewmatrix, a.projmatrix, a.campos, a.W, a.H, a.grid_x, a.grid_y, a.focal_x,
+                                   a.focal_y, a.tanfovx, a.tanfovy);
   SplatGeom o;
   if (!project_splat(p, c3, v, o)) return;
 
@@ -56,7 +46,7 @@ __global__ __launch_bounds__(256) void preprocess_fwd_kernel(PreprocessArgs a) {
   if (a.colors_precomp) {
     rgb = make_float3(a.colors_precomp[3 * idx], a.colors_precomp[3 * idx + 1], a.colors_precomp[3 * idx + 2]);
   } else {
-    float3 dir = make_float3(p.x - a.campos[0], p.y - a.campos[1], p.z - a.campos[2]);
+    float3 dir = make_float3(p.x - v.campos[0], p.y - v.campos[1], p.z - v.campos[2]);
     const float inv_len = 1.0f / sqrtf(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
     dir = make_float3(dir.x * inv_len, dir.y * inv_len, dir.z * inv_len);
     float sh[48];
