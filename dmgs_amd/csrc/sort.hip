@@ -164,6 +164,20 @@ void launch_inclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t
 // so a pass is three launches.  Blocks hold 256 x ITEMS keys; small inputs use
 // fewer items per thread so that every CU gets work.
 // ---------------------------------------------------------------------------
+// Lanes of the wave whose digit equals this lane's: one ballot per digit bit
+// (at most 8; the loop is unrolled with a uniform test of the width).
+__device__ __forceinline__ uint64_t digit_peers(uint32_t d, int nbits, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    if (b >= nbits) break;
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    peers &= bit ? bal : ~bal;
+  }
+  return peers;
+}
+
 template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, size_t n,
                                                                  int shift, int nbits,
@@ -176,21 +190,27 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * (kSortThreads * ITEMS);
   const int lane = threadIdx.x & 63;
-  // Lanes holding the same digit are found with one ballot per digit bit and
-  // counted by one LDS atomic from the group's highest lane: skewed digits
-  // (e.g. the exponent byte of depth keys) no longer serialise 64 atomics.
-#pragma unroll 4
-  for (int i = 0; i < ITEMS; ++i) {
-    const size_t idx = base + (size_t)i * kSortThreads + threadIdx.x;
-    const bool valid = idx < n;
-    const uint32_t d = valid ? (keys[idx] >> shift) & mask : 0u;
-    uint64_t peers = __ballot(valid);
-    for (int b = 0; b < nbits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
+  if (base < n) {  // uniform per block; then n >= 1
+    // every key of the thread is requested before the first is used
+    // (indices past n read the last key and count nothing)
+    uint32_t k[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const size_t idx = base + (size_t)i * kSortThreads + threadIdx.x;
+      k[i] = keys[idx < n ? idx : n - 1];
     }
-    if (valid && lane == 63 - __clzll(peers)) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
+    // Lanes holding the same digit are found with one ballot per digit bit
+    // and counted by one LDS atomic from the group's highest lane: skewed
+    // digits (e.g. the exponent byte of depth keys) do not serialise 64
+    // atomics.
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const size_t idx = base + (size_t)i * kSortThreads + threadIdx.x;
+      const bool valid = idx < n;
+      const uint32_t d = (k[i] >> shift) & mask;
+      const uint64_t peers = digit_peers(d, nbits, valid);
+      if (valid && lane == 63 - __clzll(peers)) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
+    }
   }
   __syncthreads();
   if (threadIdx.x <= mask) hist[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
@@ -259,12 +279,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const size_t idx = wbase + (size_t)i * 64 + lane;
     const bool valid = idx < n;
     const uint32_t d = (key[i] >> shift) & mask;
-    uint64_t peers = __ballot(valid);
-    for (int b = 0; b < nbits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const uint64_t bal = __ballot(bit);
-      peers &= bit ? bal : ~bal;
-    }
+    const uint64_t peers = digit_peers(d, nbits, valid);
     const uint32_t before = wcnt[wave][d];
     rank[i] = before + (uint32_t)__popcll(peers & lt);
     const int leader = 63 - __clzll(peers);
@@ -420,46 +435,63 @@ __global__ __launch_bounds__(256) void chunk_first_kernel(DupArgs a) {
   for (uint32_t c = (beg + kDupChunk - 1) / kDupChunk; c * kDupChunk < end; ++c) a.chunk_first[c] = (uint32_t)s;
 }
 
+constexpr int kDupViews = 256;  // views whose cell-grid width and base are cached in LDS
+
 __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
   __shared__ uint32_t s_end[256], s_g[256], s_tb[256];
   __shared__ int s_x0[256], s_y0[256], s_w[256], s_gx[256];
   __shared__ uint32_t s_rx[256], s_ry[256];  // multi-view: tile rectangle, [lo, hi) packed 16:16
+  __shared__ int s_vcx[kDupViews];           // multi-view: cells per row of view v
+  __shared__ uint32_t s_vtb[kDupViews];      // multi-view: key base of view v
   const uint32_t K = device_count(a.K, a.K_dev);
   const uint32_t lo_slot = blockIdx.x * kDupChunk;
   if (lo_slot >= K) return;  // uniform per block
   const uint32_t hi_slot = min(K, lo_slot + kDupChunk);
   const uint32_t P = (uint32_t)a.P;
+  const uint32_t nviews = a.views ? (P + a.per_view - 1) / a.per_view : 0u;
+  auto view_tb = [&](uint32_t v) -> uint32_t {
+    return a.view_major ? v << 12 : (uint32_t)a.views[v].cell_base;
+  };
+  for (uint32_t v = threadIdx.x; v < min(nviews, (uint32_t)kDupViews); v += 256) {
+    s_vcx[v] = cells_x(a.views[v].grid_x);
+    s_vtb[v] = view_tb(v);
+  }
+  __syncthreads();
   uint32_t first = a.chunk_first[blockIdx.x];
+  uint32_t win_begin = (first == 0) ? 0u : min(a.offsets[first - 1], K);
   while (true) {
-    // stage Gaussians first .. first+255 (those past P get an empty range)
+    // stage Gaussians first .. first+255 (those past P get an empty range);
+    // the loads of one entry are independent and issued together
     const uint32_t s = first + threadIdx.x;
     uint32_t end = 0xFFFFFFFFu, g = 0, tb = 0, rx = 0, ry = 0;
     int x0 = 0, y0 = 0, w = 1, gx = a.grid_x;
     if (s < P) {
       g = a.order[s];
-      const uint32_t beg = (s == 0) ? 0u : min(a.offsets[s - 1], K);
+      const uint32_t prev_end = a.offsets[s > 0 ? s - 1 : 0];  // unconditional: no branch between the loads
       end = min(a.offsets[s], K);
+      const uint2 pr = a.views ? a.rects[s] : make_uint2(0u, 0u);
+      const uint32_t beg = (s == 0) ? 0u : min(prev_end, K);
       if (end > beg && g < P) {
-        const ViewDesc* v = a.views ? &a.views[g / a.per_view] : nullptr;
         int2 rmin, rmax;
-        if (v) {  // depth-ordered rectangle from the gather step
-          const uint2 pr = a.rects[s];
+        if (a.views) {  // depth-ordered rectangle from the gather step; per binning cell of 4x4 tiles
+          const uint32_t v = g / a.per_view;
           unpack_rect(pr, rmin, rmax);
-          gx = v->grid_x;
+          rx = (uint32_t)rmin.x | ((uint32_t)rmax.x << 16);
+          ry = (uint32_t)rmin.y | ((uint32_t)rmax.y << 16);
+          if (v < (uint32_t)kDupViews) {
+            gx = s_vcx[v];
+            tb = s_vtb[v];
+          } else {
+            gx = cells_x(a.views[v].grid_x);
+            tb = view_tb(v);
+          }
+          x0 = rmin.x >> DMGS_CELL_SHIFT;
+          y0 = rmin.y >> DMGS_CELL_SHIFT;
+          w = max(1, ((rmax.x - 1) >> DMGS_CELL_SHIFT) + 1 - x0);
         } else {
           const float4 s0 = a.splats[(size_t)DMGS_SPLAT_F4 * g];
           const int r = __float_as_int(a.splats[(size_t)DMGS_SPLAT_F4 * g + 2].z);
           tile_rect(make_float2(s0.x, s0.y), r, gx, a.grid_y, rmin, rmax);
-        }
-        if (v) {  // emit per binning cell of 4x4 tiles
-          rx = (uint32_t)rmin.x | ((uint32_t)rmax.x << 16);
-          ry = (uint32_t)rmin.y | ((uint32_t)rmax.y << 16);
-          tb = a.view_major ? (g / a.per_view) << 12 : (uint32_t)v->cell_base;
-          x0 = rmin.x >> DMGS_CELL_SHIFT;
-          y0 = rmin.y >> DMGS_CELL_SHIFT;
-          w = max(1, ((rmax.x - 1) >> DMGS_CELL_SHIFT) + 1 - x0);
-          gx = cells_x(gx);
-        } else {
           x0 = rmin.x;
           y0 = rmin.y;
           w = max(1, rmax.x - rmin.x);
@@ -476,7 +508,6 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
     s_rx[threadIdx.x] = rx;
     s_ry[threadIdx.x] = ry;
     __syncthreads();
-    const uint32_t win_begin = (first == 0) ? 0u : min(a.offsets[first - 1], K);
     const uint32_t win_end = min(s_end[255], hi_slot);
     const uint32_t o_begin = max(lo_slot, win_begin);
     for (uint32_t o = o_begin + threadIdx.x; o < win_end; o += 256) {
@@ -502,6 +533,7 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
       a.values[o] = s_g[lo];
     }
     const bool more = win_end < hi_slot && first + 256 < P;
+    win_begin = s_end[255];  // the next window starts where this one ends
     __syncthreads();
     if (!more) break;
     first += 256;
