@@ -240,7 +240,12 @@ __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_
     const int s = (int)threadIdx.x;
     const int q = BACKWARD ? cursor - 1 - s : cursor + s;
     const bool valid = s < CH && (BACKWARD ? q >= 0 : q < limit);
-    const uint32_t key = valid ? a.list_keys[c.range.x + q] : 0u;
+    // The entry id is requested with the key, both without a branch (lanes
+    // past the list read its first entry and ignore it): one memory round
+    // trip, not two.
+    const uint32_t at = c.range.x + (valid ? (uint32_t)q : 0u);
+    const uint32_t key = a.list_keys[at];
+    const uint32_t id = a.point_list[at];
     const bool member = valid && ((key >> DMGS_CELL_KEY_BITS) & c.bit);
     const uint64_t bal = __ballot(member);
     if (lane == 0) s_wc[wave] = (uint32_t)__popcll(bal);
@@ -254,7 +259,7 @@ __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_
     }
     if (member) {
       const int slot = before + (int)__popcll(bal & lt);
-      stage_splat(L, slot, a.splats, a.point_list[c.range.x + q], a.P, c.rows);
+      stage_splat(L, slot, a.splats, id, a.P, c.rows);
       s_pos[slot] = (uint32_t)q;
     }
     run += total;
