@@ -42,7 +42,7 @@ class DmgsBatch(ctypes.Structure):
         ("num_gaussians", c_int64), ("scale_modifier", c_float), ("variant", c_int32), ("total_tiles", c_int32),
         ("total_cells", c_int32), ("total_pixels", c_int64), ("capacity", c_int64),
         ("state", c_void_p), ("lists", c_void_p), ("bwd_ws", c_void_p),
-        ("cell_mode", c_int32), ("max_view_cells", c_int32),
+        ("cell_mode", c_int32), ("max_view_cells", c_int32), ("dens", c_void_p),
     ]
 
 
@@ -63,6 +63,9 @@ _SIGNATURES = {
     "dmgs_geometry_backward": (c_int32, [ctypes.POINTER(DmgsSettings), c_int64] + [c_void_p] * 17 + [c_void_p]),
     "dmgs_backward": (c_int32, [ctypes.POINTER(DmgsSettings), c_int64, c_int64] + [c_void_p] * 21 + [c_void_p]),
     "dmgs_mark_visible": (c_int32, [c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dmgs_ply_rows_to_params": (c_int32, [c_void_p, c_int64, c_int32, ctypes.POINTER(c_int32), c_void_p, c_void_p]),
+    "dmgs_params_to_ply_rows": (c_int32, [c_void_p, c_int64, c_int32, ctypes.POINTER(c_int32), c_void_p, c_void_p]),
+    "dmgs_image_rgb8": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
     "dmgs_sort_scratch_size": (c_size_t, [c_int64]),
     "dmgs_sort_pairs": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p]),
     "dmgs_bind_forward": (c_int32, [c_int64, c_void_p, c_int64, c_void_p, c_int32, c_void_p, c_int32, c_void_p,

@@ -132,6 +132,7 @@ class BatchedRenderer:
         self.pixel_base = [v.pixel_base for v in self.views]
         self.K_cap = 0
         self.state = self.lists = self.bwd_ws = None
+        self.dens = None
         self._offsets = None
 
     # ---- planning ---------------------------------------------------------
@@ -156,6 +157,7 @@ class BatchedRenderer:
         d.state = self.state.data_ptr()
         d.lists = None if self.lists is None else self.lists.data_ptr()
         d.bwd_ws = None if self.bwd_ws is None else self.bwd_ws.data_ptr()
+        d.dens = None if self.dens is None else self.dens.data_ptr()
         return d
 
     def count_instances(self, params):
@@ -239,6 +241,29 @@ class BatchedRenderer:
                                                N.ptr(grads), stages, N.stream_handle(self.device)),
                 "dmgs_batched_backward")
         return grads
+
+    # ---- densification statistics ------------------------------------------
+    def track_densification(self, enable=True):
+        """Let every backward accumulate, per Gaussian over the views where it
+        is visible and in view order, the statistics the reference's training
+        loop gathers one view at a time (gaussian_model.py:405-407,
+        train.py:116-118): |dL/dmean2D| in NDC, the number of views and the
+        largest screen radius.  Starts from zero."""
+        if enable and self.dens is None:
+            self.dens = torch.zeros(3, self.P, dtype=torch.float32, device=self.device)
+        elif not enable:
+            self.dens = None
+
+    def densification_stats(self):
+        """(xyz_gradient_accum (P, 1), denom (P, 1), max_radii2D (P,)) views
+        of the accumulated statistics."""
+        if self.dens is None:
+            raise RuntimeError("track_densification() is off")
+        return self.dens[0].view(-1, 1), self.dens[1].view(-1, 1), self.dens[2]
+
+    def reset_densification_stats(self):
+        if self.dens is not None:
+            self.dens.zero_()
 
     # ---- results ----------------------------------------------------------
     def image(self, packed, v):

@@ -683,3 +683,53 @@ int dmgs_faces_in_frustum(int64_t V, const float* verts, int64_t F, const int32_
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Scene and frame conversions
+// ---------------------------------------------------------------------------
+int dmgs_ply_rows_to_params(const float* rows, int64_t P, int32_t row_floats, const int32_t channel_cols[59],
+                            float* params, dmgs_stream_t stream) {
+  if (P < 0 || P > (int64_t)0x7fffffff) return fail(DMGS_EINVAL, "invalid P");
+  if (row_floats <= 0 || row_floats > dmgs::kMaxRowFloats)
+    return fail(DMGS_EINVAL, "row_floats must be in [1, %d]", dmgs::kMaxRowFloats);
+  if (P == 0) return DMGS_OK;
+  if (!rows || !channel_cols || !params) return fail(DMGS_EINVAL, "null pointer");
+  dmgs::ChannelMap map{};
+  map.channels = DMGS_MV_CHANNELS;
+  for (int c = 0; c < DMGS_MV_CHANNELS; ++c) {
+    if (channel_cols[c] < -1 || channel_cols[c] >= row_floats)
+      return fail(DMGS_EINVAL, "channel %d: column %d outside the row", c, channel_cols[c]);
+    map.col[c] = channel_cols[c];
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  dmgs::launch_rows_to_channels(rows, P, row_floats, map, params, st);
+  return check_launch("ply rows to params", st, false);
+}
+
+int dmgs_params_to_ply_rows(const float* params, int64_t P, int32_t row_floats, const int32_t* column_channels,
+                            float* rows, dmgs_stream_t stream) {
+  if (P < 0 || P > (int64_t)0x7fffffff) return fail(DMGS_EINVAL, "invalid P");
+  if (row_floats <= 0 || row_floats > dmgs::kMaxRowFloats)
+    return fail(DMGS_EINVAL, "row_floats must be in [1, %d]", dmgs::kMaxRowFloats);
+  if (P == 0) return DMGS_OK;
+  if (!rows || !column_channels || !params) return fail(DMGS_EINVAL, "null pointer");
+  dmgs::ChannelMap map{};
+  map.channels = row_floats;
+  for (int j = 0; j < row_floats; ++j) {
+    if (column_channels[j] < -1 || column_channels[j] >= DMGS_MV_CHANNELS)
+      return fail(DMGS_EINVAL, "column %d: channel %d outside the scene layout", j, column_channels[j]);
+    map.col[j] = column_channels[j];
+  }
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  dmgs::launch_channels_to_rows(params, P, row_floats, map, rows, st);
+  return check_launch("params to ply rows", st, false);
+}
+
+int dmgs_image_rgb8(const float* chw, int32_t H, int32_t W, uint8_t* hwc, dmgs_stream_t stream) {
+  if (H < 0 || W < 0) return fail(DMGS_EINVAL, "invalid image size");
+  if ((int64_t)H * W == 0) return DMGS_OK;
+  if (!chw || !hwc) return fail(DMGS_EINVAL, "null pointer");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  dmgs::launch_rgb8(chw, (int64_t)H * W, hwc, st);
+  return check_launch("image rgb8", st, false);
+}

@@ -31,7 +31,8 @@ static_assert(offsetof(dmgs_view, pixel_base) == offsetof(dmgs::ViewDesc, pixel_
 static_assert(offsetof(dmgs_view, tanfovx) == offsetof(dmgs::ViewDesc, tanfovx), "dmgs_view layout");
 static_assert(offsetof(dmgs_view, width) == offsetof(dmgs::ViewDesc, W), "dmgs_view layout");
 static_assert(offsetof(dmgs_view, bg) == offsetof(dmgs::ViewDesc, bg), "dmgs_view layout");
-static_assert(sizeof(dmgs_batch) == 96 && offsetof(dmgs_batch, capacity) == 56, "dmgs_batch layout");
+static_assert(sizeof(dmgs_batch) == 104 && offsetof(dmgs_batch, capacity) == 56 && offsetof(dmgs_batch, dens) == 96,
+              "dmgs_batch layout");
 static_assert(offsetof(dmgs_view, cell_base) == offsetof(dmgs::ViewDesc, cell_base), "dmgs_view layout");
 
 namespace {
@@ -121,6 +122,7 @@ int check_batch(const dmgs_batch* b) {
     return bfail(DMGS_EINVAL, "cell_mode 1 needs at most 16 views and 1..4096 cells per view");
   if (b->sh_degree < 0 || b->sh_degree > 3) return bfail(DMGS_EINVAL, "sh_degree must be in [0, 3]");
   if (!b->views || !b->state) return bfail(DMGS_EINVAL, "views and state are required");
+  if (b->dens && (reinterpret_cast<uintptr_t>(b->dens) & 3u)) return bfail(DMGS_EINVAL, "dens must be 4-byte aligned");
   const auto misaligned = [](const void* p) { return p && (reinterpret_cast<uintptr_t>(p) & 255u) != 0; };
   if (misaligned(b->state) || misaligned(b->lists) || misaligned(b->bwd_ws))
     return bfail(DMGS_EINVAL, "workspaces must be 256-byte aligned");
@@ -159,6 +161,7 @@ dmgs::MVArgs mv_args(const dmgs_batch* b, const Ctx& c, const float* params) {
   a.order = at<uint32_t>(b->state, c.S.order_a);
   a.wave_counts = at<uint32_t>(b->state, c.S.wave_counts);
   a.gacc = static_cast<float*>(b->bwd_ws);
+  a.dens = b->dens;
   return a;
 }
 

@@ -153,9 +153,17 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
   for (int k = 0; k < 48; ++k) dsh[k] = 0.f;
   float dop = 0.f;
   bool any = false;
+  // densification statistics (gaussian_model.py:405-407, train.py:116-118)
+  float d_acc = 0.f, d_cnt = 0.f, d_maxr = 0.f;
+  if (a.dens) {
+    d_acc = a.dens[g];
+    d_cnt = a.dens[(size_t)a.P + g];
+    d_maxr = a.dens[2 * (size_t)a.P + g];
+  }
   for (int v = 0; v < a.B; ++v) {
     const size_t i = (size_t)v * a.P + g;
-    if (a.radii[i] <= 0) continue;
+    const int radius = a.radii[i];
+    if (radius <= 0) continue;
     any = true;
     float* gr = a.gacc + i * DMGS_GACC_STRIDE;
     const float4 g03 = *reinterpret_cast<const float4*>(gr);
@@ -167,6 +175,11 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
     reinterpret_cast<float4*>(gr)[0] = z;
     reinterpret_cast<float4*>(gr)[1] = z;
     reinterpret_cast<float4*>(gr)[2] = z;
+    if (a.dens) {
+      d_acc += sqrtf(g03.x * g03.x + g03.y * g03.y);
+      d_cnt += 1.0f;
+      d_maxr = fmaxf(d_maxr, (float)radius);
+    }
     const float3 dm = ewa_backward(G.p, G.c3, vp, g03.x, g03.y, g03.z, g03.w, g47.x, dc);
     float b[16];
     float3 gm;
@@ -196,6 +209,11 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
                       (dq.z - G.q.z * qd) * G.rq_inv, (dq.w - G.q.w * qd) * G.rq_inv);
   }
   const size_t P = (size_t)a.P;
+  if (a.dens) {
+    a.dens[g] = d_acc;
+    a.dens[P + g] = d_cnt;
+    a.dens[2 * P + g] = d_maxr;
+  }
   float* gd = a.grads;
   auto put = [&](int c, float val) { gd[(size_t)c * P + g] = val; };
   put(DMGS_MV_XYZ, dmean.x);

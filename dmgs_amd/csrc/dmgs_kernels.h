@@ -109,6 +109,7 @@ struct MVArgs {
   uint32_t* wave_counts;     // mv_wave_count_words(P, B): visible entries per (view, block, wave)
   float* gacc;               // (B*P, 16) blend-backward accumulator; zero between backward passes
   float* grads;              // (59, P)
+  float* dens;               // optional (3, P): densification statistics (see dmgs_batch)
 };
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s);
 size_t mv_wave_count_words(int P, int B);
@@ -238,5 +239,21 @@ void launch_dssim_forward(const float* x, const float* y, int C, int group, int 
                           float lam, float* coef, float* partials, float* out, hipStream_t s);
 void launch_dssim_backward(const float* x, const float* y, const float* coef, int C, int group, int H, int W,
                            const float* w11, float lam, const float* dloss, float* dx, hipStream_t s);
+
+// ---- scene / frame conversions ----
+constexpr int kMaxRowFloats = 128;   // properties per PLY vertex row
+constexpr int kMaxChannels = 128;
+struct ChannelMap {
+  int channels;                      // entries used in col
+  int col[kMaxChannels];             // source index per destination (-1: zero)
+};
+// params[c][g] = rows[g][map.col[c]] for c < map.channels (59 scene channels)
+void launch_rows_to_channels(const float* rows, int64_t P, int row_len, const ChannelMap& map, float* params,
+                             hipStream_t s);
+// rows[g][j] = params[map.col[j]][g] for j < row_len
+void launch_channels_to_rows(const float* params, int64_t P, int row_len, const ChannelMap& map, float* rows,
+                             hipStream_t s);
+// (3, H*W) float -> H*W*3 bytes of uint8(clamp(x, 0, 1) * 255)
+void launch_rgb8(const float* chw, int64_t plane, uint8_t* hwc, hipStream_t s);
 
 }  // namespace dmgs

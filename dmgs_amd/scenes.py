@@ -65,6 +65,28 @@ class FreeGaussians:
     def get_opacity(self):
         return torch.sigmoid(self._opacity)
 
+    # ---- densification statistics (the consumer of radii and the NDC
+    # means2D gradient; scene/gaussian_model.py:405-407, train.py:116-118) ----
+    def _stats(self):
+        if getattr(self, "xyz_gradient_accum", None) is None or self.xyz_gradient_accum.shape[0] != self.num:
+            dev = self._xyz.device
+            self.xyz_gradient_accum = torch.zeros((self.num, 1), device=dev)
+            self.denom = torch.zeros((self.num, 1), device=dev)
+            self.max_radii2D = torch.zeros((self.num,), device=dev)
+
+    def add_densification_stats(self, viewspace_point_tensor, update_filter):
+        """Accumulate |dL/dmean2D[:2]| (NDC) and the view count of the visible
+        Gaussians of one rendered view."""
+        self._stats()
+        self.xyz_gradient_accum[update_filter] += torch.norm(viewspace_point_tensor.grad[update_filter, :2], dim=-1,
+                                                             keepdim=True)
+        self.denom[update_filter] += 1
+
+    def update_max_radii2D(self, radii, visibility_filter):
+        """Keep the largest screen radius of each visible Gaussian."""
+        self._stats()
+        self.max_radii2D[visibility_filter] = torch.max(self.max_radii2D[visibility_filter], radii[visibility_filter])
+
     def get_covariance(self, scaling_modifier=1.0):
         """build_covariance_from_scaling_rotation (scene/gaussian_model.py:27-31)
         with the raw rotation, as the reference passes ``self._rotation``."""

@@ -42,9 +42,10 @@ class Request:
 
     _ids = itertools.count()
 
-    def __init__(self, scene, camera, bg=(1.0, 1.0, 1.0), arrival=None):
+    def __init__(self, scene, camera, bg=(1.0, 1.0, 1.0), arrival=None, scaling_modifier=1.0):
         self.id = next(self._ids)
         self.scene, self.camera = scene, camera
+        self.scaling_modifier = float(scaling_modifier)
         self.bg = torch.as_tensor(bg, dtype=torch.float32)
         self.arrival = arrival
         self.image = None
@@ -71,7 +72,8 @@ class _Batch:
         self.params = params
         with torch.cuda.stream(stream):
             self.r = BT.BatchedRenderer([q.camera for q in requests], [q.bg for q in requests], params.shape[1],
-                                        sh_degree=sh, device=server.device, variant=server.variant)
+                                        sh_degree=sh, scale_modifier=requests[0].scaling_modifier,
+                                        device=server.device, variant=server.variant)
             self.r._alloc_state()
             self.images = self.r.new_images()
             self.count = torch.empty(2, dtype=torch.int32, pin_memory=True)  # instances, status word
@@ -172,7 +174,7 @@ class RenderServer:
         for q in itertools.islice(self.pending, 1, None):
             if len(group) >= self.max_views:
                 break
-            if q.scene == head.scene:
+            if q.scene == head.scene and q.scaling_modifier == head.scaling_modifier:
                 group.append(q)
         # shrink until the batch fits the free budget (the head alone always fits an empty server)
         while group:

@@ -306,6 +306,15 @@ typedef struct dmgs_batch {
    * re-run with cell_mode 0). */
   int32_t cell_mode;
   int32_t max_view_cells; /* largest cell count of one view (dmgs_batched_prepare_views) */
+  /* Optional (3 x P floats, or NULL): densification statistics that a
+   * backward with the GEOMETRY stage accumulates, per Gaussian and per view
+   * where it is visible, in view order — the consumer of
+   * gaussian_model.py:405-407 and train.py:116-118:
+   *   dens[g]       += |dL/dmean2D (NDC) of that view|_2   (xyz_gradient_accum)
+   *   dens[P + g]   += 1                                   (denom)
+   *   dens[2P + g]   = max(dens[2P + g], radius in that view)  (max_radii2D)
+   * The caller zeroes it when it resets the statistics. */
+  float* dens;
 } dmgs_batch;
 /* Byte offsets inside `lists` of the sorted keys and entry ids that a forward
  * of this batch leaves (they depend on cell_mode and max_view_cells). */
@@ -331,6 +340,25 @@ int dmgs_batched_forward(const dmgs_batch* batch, const float* params, float* im
  * (packed like images), using the state and lists the forward left. */
 int dmgs_batched_backward(const dmgs_batch* batch, const float* params, const float* dL_dimages, float* grads,
                           int32_t stages, dmgs_stream_t stream);
+
+/* ---- Scene and frame conversions ---- */
+/* PLY vertex rows (P x row_floats float32, row-major, as the 3DGS writer of
+ * scene/gaussian_model.py:191-210 lays them out) -> the 59-channel
+ * component-major scene parameters of the batched path.  channel_cols[c] is
+ * the row column holding channel c (xyz 0-2, log-scale 3-5, raw quaternion
+ * 6-9, opacity logit 10, SH coefficient k of colour c at 11 + 3k + c), or -1
+ * for a channel the file lacks (written as 0; gaussian_model.py:214-250 reads
+ * the same properties by name).  row_floats <= 128. */
+int dmgs_ply_rows_to_params(const float* rows, int64_t P, int32_t row_floats, const int32_t channel_cols[59],
+                            float* params, dmgs_stream_t stream);
+/* The reverse: rows[g][j] = params[column_channels[j]][g], 0 where it is -1
+ * (the writer's normals). */
+int dmgs_params_to_ply_rows(const float* params, int64_t P, int32_t row_floats, const int32_t* column_channels,
+                            float* rows, dmgs_stream_t stream);
+/* A (3, H, W) float image -> H x W x 3 bytes of uint8(clamp(x, 0, 1) * 255),
+ * truncating: the frame the viewer protocol sends (train.py:60,
+ * gaussian_renderer/network_gui.py:51-55). */
+int dmgs_image_rgb8(const float* chw, int32_t H, int32_t W, uint8_t* hwc, dmgs_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -97,8 +97,8 @@ def _desc(**kw):
 
 
 def test_batch_descriptor_layout():
-    # struct dmgs_batch: 2 pointers, 2 int32, int64, float, 3 int32, 2 int64, 3 pointers, 2 int32
-    assert ctypes.sizeof(N.DmgsBatch) == 96
+    # struct dmgs_batch: 2 pointers, 2 int32, int64, float, 3 int32, 2 int64, 3 pointers, 2 int32, pointer
+    assert ctypes.sizeof(N.DmgsBatch) == 104 and N.DmgsBatch.dens.offset == 96
     assert N.DmgsBatch.capacity.offset == 56 and N.DmgsBatch.state.offset == 64
     assert N.DmgsBatch.cell_mode.offset == 88 and N.DmgsBatch.max_view_cells.offset == 92
     assert N.DmgsBatch.total_cells.offset == 44 and N.DmgsView.cell_base.offset == 196
@@ -126,6 +126,8 @@ def test_entry_points_validate_before_launch():
     assert lib.dmgs_batched_forward(_desc(total_cells=0), fake, fake, A, None) == -1
     assert lib.dmgs_batched_forward(_desc(total_cells=5), fake, fake, A, None) == -1  # more cells than tiles
     # view-major lists: at most 16 views of at most 4096 cells
+    assert lib.dmgs_batched_forward(_desc(dens=0x1002), fake, fake, A, None) == -1
+    assert "dens" in N.last_error()
     assert lib.dmgs_batched_forward(_desc(cell_mode=2), fake, fake, A, None) == -1
     assert "cell_mode" in N.last_error()
     assert lib.dmgs_batched_forward(_desc(cell_mode=1, num_views=17), fake, fake, A, None) == -1

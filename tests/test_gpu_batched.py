@@ -351,3 +351,40 @@ def test_view_major_far_depth_falls_back():
     assert not r.overflowed()
     r0, _ = _batched(model, cams, bgs, cell_mode=0)
     assert torch.equal(img, r0.forward(params))
+
+
+def test_densification_stats_match_per_view_consumer():
+    """The statistics a batched backward accumulates equal the reference
+    consumer (add_densification_stats + the max_radii2D update) run after
+    each single-view render + backward, views in order: the norm sums within
+    1e-4 of their largest value, view counts and radii exactly."""
+    print("case: densification statistics", flush=True)
+    model = S.free_scene(20000, seed=12, layout="mixture", shell_radius=3.0)
+    cams = _cameras([(160, 120), (128, 128), (200, 100)], seed=4)
+    bgs = _bgs(len(cams))
+    r, params = _batched(model, cams, bgs)
+    r.track_densification()
+    dL = _dL(r, seed=5)
+    for v in range(r.B):  # keep threshold ties out of the comparison
+        r.image(dL, v)[:, _ambiguous(model, cams[v], bgs[v]).to(r.device)] = 0.0
+    r.forward(params)
+    r.backward(params, dL)
+    r.backward(params, dL)  # a second step keeps accumulating
+    torch.cuda.synchronize()
+    acc, den, mr = r.densification_stats()
+    m = model.to("cuda")
+    pipe = PipelineParams()
+    for _ in range(2):
+        for v, (cam, bg) in enumerate(zip(cams, bgs)):
+            mv = model.to("cuda").requires_grad_(True)
+            out = render(cam, mv, pipe, bg.cuda())
+            (out["render"] * r.image(dL, v)).sum().backward()
+            vis = out["visibility_filter"]
+            m.update_max_radii2D(out["radii"], vis)
+            m.add_densification_stats(out["viewspace_points"], vis)
+    assert torch.equal(den, m.denom)
+    assert torch.equal(mr, m.max_radii2D)
+    err = (acc - m.xyz_gradient_accum).abs().max().item()
+    scale = m.xyz_gradient_accum.abs().max().item()
+    print(f"max |accum diff| {err:.3g} of {scale:.3g}", flush=True)
+    assert scale > 0 and err <= 1e-4 * scale
