@@ -398,6 +398,9 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
   const TileCtx c = tile_ctx<16>(a);
   const float pxf = (float)c.px;
   const int len = (int)(c.range.y - c.range.x);
+  float pyf[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) pyf[k] = (float)(c.py + 4 * k);
   float T[4], C0[4], C1[4], C2[4];
   bool done[4];
   uint32_t last_contributor[4];
@@ -436,27 +439,25 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
         position = (uint32_t)(base + j);
       }
       const float dx = xy.x - pxf;
+      const float4 col = L.rgb_id[j];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (!((reach >> k) & 1u) || __ballot(!done[k]) == 0) continue;  // wave-uniform
-        if (done[k]) continue;
-        const float dy = xy.y - (float)(c.py + 4 * k);
+        // the per-pixel tests become selects: no branch per pixel and splat
+        const float dy = xy.y - pyf[k];
         const float power = splat_power(co, dx, dy);
-        if (power > 0.0f) continue;
         const float alpha = fminf(0.99f, co.w * __expf(power));
-        if (alpha < 1.0f / 255.0f) continue;
         const float test_T = T[k] * (1.0f - alpha);
-        if (test_T < 0.0001f) {
-          done[k] = true;
-          continue;
-        }
-        const float4 col = L.rgb_id[j];
+        const bool ok = !done[k] && power <= 0.0f && alpha >= 1.0f / 255.0f;
+        const bool stop = ok && test_T < 0.0001f;
+        const bool take = ok && !stop;
         const float w = alpha * T[k];
-        C0[k] += col.x * w;
-        C1[k] += col.y * w;
-        C2[k] += col.z * w;
-        T[k] = test_T;
-        last_contributor[k] = position + 1;
+        C0[k] = take ? fmaf(col.x, w, C0[k]) : C0[k];
+        C1[k] = take ? fmaf(col.y, w, C1[k]) : C1[k];
+        C2[k] = take ? fmaf(col.z, w, C2[k]) : C2[k];
+        T[k] = take ? test_T : T[k];
+        last_contributor[k] = take ? position + 1 : last_contributor[k];
+        done[k] = done[k] || stop;
       }
     }
   }
