@@ -401,19 +401,19 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
   float pyf[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) pyf[k] = (float)(c.py + 4 * k);
+  // T > 0 while a pixel blends; a finished pixel (saturated, or outside the
+  // image) keeps -T, so one compare tells both
   float T[4], C0[4], C1[4], C2[4];
-  bool done[4];
   uint32_t last_contributor[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    done[k] = !(c.px < c.W && c.py + 4 * k < c.H);
-    T[k] = 1.0f;
+    T[k] = (c.px < c.W && c.py + 4 * k < c.H) ? 1.0f : -1.0f;
     C0[k] = C1[k] = C2[k] = 0.f;
     last_contributor[k] = 0;
   }
   int cursor = 0;
   while (cursor < len) {
-    if (__syncthreads_count(done[0] & done[1] & done[2] & done[3]) == NT) break;
+    if (__syncthreads_count(T[0] < 0.f && T[1] < 0.f && T[2] < 0.f && T[3] < 0.f) == NT) break;
     const int base = cursor;
     int n;
     if constexpr (CELLS) {
@@ -442,22 +442,21 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
       const float4 col = L.rgb_id[j];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        if (!((reach >> k) & 1u) || __ballot(!done[k]) == 0) continue;  // wave-uniform
+        if (!((reach >> k) & 1u) || __ballot(T[k] > 0.f) == 0) continue;  // wave-uniform
         // the per-pixel tests become selects: no branch per pixel and splat
         const float dy = xy.y - pyf[k];
         const float power = splat_power(co, dx, dy);
         const float alpha = fminf(0.99f, co.w * __expf(power));
         const float test_T = T[k] * (1.0f - alpha);
-        const bool ok = !done[k] && power <= 0.0f && alpha >= 1.0f / 255.0f;
+        const bool ok = T[k] > 0.f && power <= 0.0f && alpha >= 1.0f / 255.0f;
         const bool stop = ok && test_T < 0.0001f;
         const bool take = ok && !stop;
         const float w = alpha * T[k];
         C0[k] = take ? fmaf(col.x, w, C0[k]) : C0[k];
         C1[k] = take ? fmaf(col.y, w, C1[k]) : C1[k];
         C2[k] = take ? fmaf(col.z, w, C2[k]) : C2[k];
-        T[k] = take ? test_T : T[k];
+        T[k] = take ? test_T : (stop ? -T[k] : T[k]);
         last_contributor[k] = take ? position + 1 : last_contributor[k];
-        done[k] = done[k] || stop;
       }
     }
   }
@@ -466,12 +465,13 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
     if (!(c.px < c.W && c.py + 4 * k < c.H)) continue;
     const size_t pix = c.pix + (size_t)(4 * k) * c.W;
     const size_t pid = pix - c.img / 3;
-    a.final_T[pix] = T[k];
+    const float Tk = fabsf(T[k]);
+    a.final_T[pix] = Tk;
     a.n_contrib[pix] = last_contributor[k];
     float* out = a.out_color + c.img;
-    out[pid] = C0[k] + T[k] * c.bg0;
-    out[c.plane + pid] = C1[k] + T[k] * c.bg1;
-    out[2 * c.plane + pid] = C2[k] + T[k] * c.bg2;
+    out[pid] = C0[k] + Tk * c.bg0;
+    out[c.plane + pid] = C1[k] + Tk * c.bg1;
+    out[2 * c.plane + pid] = C2[k] + Tk * c.bg2;
   }
 }
 
@@ -620,13 +620,20 @@ template <int BATCH, bool CELLS>
 __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   constexpr int NT = 64;
   __shared__ SplatLDS<BATCH> L;
-  __shared__ float s_acc[BATCH][DMGS_GACC_USED];
+  // Two accumulator sets: a batch's global atomics are issued after the next
+  // batch has been staged, so no staging load waits behind them (the memory
+  // counter is in order) and they complete while that batch is computed.
+  __shared__ float s_acc[2][BATCH][DMGS_GACC_USED];
+  __shared__ uint32_t s_gid[2][BATCH];
   __shared__ uint32_t s_pos[CELLS ? BATCH : 1];
   __shared__ uint32_t s_wc[1];
   const TileCtx c = tile_ctx<16>(a);  // pixel k = 0 of this lane: (lx, ly), ly in 0..3
   const unsigned lane = threadIdx.x & 63;
   const float pxf = (float)c.px;
   const float ddelx_dx = 0.5f * (float)c.W, ddely_dy = 0.5f * (float)c.H;  // d pixel / d NDC
+  float pyf[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) pyf[k] = (float)(c.py + 4 * k);
 
   float T[4], T_final[4], g0[4], g1[4], g2[4], bg_term[4];
   float acc0[4], acc1[4], acc2[4], last_alpha[4], lc0[4], lc1[4], lc2[4];
@@ -659,28 +666,42 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   const int todo_total = min((int)(c.range.y - c.range.x), (int)m);
 
   const unsigned row_lane = lane & 15;
+  // one batch's accumulated values -> one global atomic per (splat, value)
+  auto flush = [&](int buf, int cnt) {
+    for (int i = threadIdx.x; i < cnt * DMGS_GACC_USED; i += NT) {
+      const int slot = i / DMGS_GACC_USED, q = i - slot * DMGS_GACC_USED;
+      const float scale = q == 0 ? ddelx_dx : (q == 1 ? ddely_dy : (q < 5 ? -0.5f : 1.0f));
+      const float val = s_acc[buf][slot][q] * scale;
+      if (val != 0.f) unsafeAtomicAdd(a.gacc + (size_t)s_gid[buf][slot] * DMGS_GACC_STRIDE + q, val);
+    }
+  };
+  int buf = 0, prev_n = 0;
   int cursor = todo_total;  // list positions below cursor are still to do
   while (cursor > 0) {
     __syncthreads();
     const int top = cursor;
     int n;
-    if constexpr (CELLS) {
-      for (int s = threadIdx.x; s < BATCH; s += NT) {
+    for (int s = threadIdx.x; s < BATCH; s += NT) {
 #pragma unroll
-        for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
-      }
+      for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[buf][s][k] = 0.f;
+    }
+    if constexpr (CELLS) {
       n = stage_cell_batch<NT, BATCH, true>(L, s_pos, s_wc, a, c, cursor, 0);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = top - 1 - s;
         if (pos >= 0) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
-#pragma unroll
-        for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
       __syncthreads();
       n = min(BATCH, top);
       cursor -= n;
     }
+    for (int s = threadIdx.x; s < n; s += NT) s_gid[buf][s] = __float_as_uint(L.rgb_id[s].w);
+    // Every staging load has landed here; saying so keeps the compiler from
+    // waiting later (behind the atomics below) for a load it cannot rule out.
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    flush(buf ^ 1, prev_n);  // the previous batch's atomics overlap this batch
+    __syncthreads();
     for (int j = 0; j < n; ++j) {
       const uint32_t reach = __builtin_amdgcn_readfirstlane(L.mask[j]);
       if (reach == 0u) continue;
@@ -692,6 +713,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       }
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
+      const float4 col = L.rgb_id[j];
       float v[DMGS_GACC_USED];
 #pragma unroll
       for (int q = 0; q < DMGS_GACC_USED; ++q) v[q] = 0.f;
@@ -699,13 +721,12 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (!((reach >> k) & 1u)) continue;  // wave-uniform
-        const float dx = xy.x - pxf, dy = xy.y - (float)(c.py + 4 * k);
+        const float dx = xy.x - pxf, dy = xy.y - pyf[k];
         const float power = splat_power(co, dx, dy);
         const float G = __expf(power);
         const float alpha = fminf(0.99f, co.w * G);
         if (contributor < last_contributor[k] && power <= 0.0f && alpha >= 1.0f / 255.0f) {
           any = true;
-          const float4 col = L.rgb_id[j];
           const float inv_one_m = __builtin_amdgcn_rcpf(1.f - alpha);
           T[k] = T[k] * inv_one_m;
           const float dchannel_dcolor = alpha * T[k];
@@ -737,19 +758,13 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       }
       if (__ballot(any) == 0) continue;
       const float mine = row_sums9(v, row_lane);
-      if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[j][row_lane], mine);
+      if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[buf][j][row_lane], mine);
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n * DMGS_GACC_USED; i += NT) {
-      const int slot = i / DMGS_GACC_USED, q = i - slot * DMGS_GACC_USED;
-      const float scale = q == 0 ? ddelx_dx : (q == 1 ? ddely_dy : (q < 5 ? -0.5f : 1.0f));
-      const float val = s_acc[slot][q] * scale;
-      if (val != 0.f) {
-        const uint32_t gid = __float_as_uint(L.rgb_id[slot].w);
-        unsafeAtomicAdd(a.gacc + (size_t)gid * DMGS_GACC_STRIDE + q, val);
-      }
-    }
+    prev_n = n;
+    buf ^= 1;
   }
+  __syncthreads();
+  flush(buf ^ 1, prev_n);
 }
 
 template <int SUB, int BATCH>
