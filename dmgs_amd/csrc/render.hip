@@ -427,8 +427,10 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
       n = min(BATCH, len - base);
       cursor += n;
     }
+    uint32_t mask_next = n > 0 ? L.mask[0] : 0u;  // read one splat ahead
     for (int j = 0; j < n; ++j) {
-      const uint32_t reach = __builtin_amdgcn_readfirstlane(L.mask[j]);
+      const uint32_t reach = __builtin_amdgcn_readfirstlane(mask_next);
+      if (j + 1 < n) mask_next = L.mask[j + 1];
       if (reach == 0u) continue;
       const float2 xy = L.xy[j];
       const float4 co = L.con_o[j];
@@ -702,8 +704,10 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     flush(buf ^ 1, prev_n);  // the previous batch's atomics overlap this batch
     __syncthreads();
+    uint32_t mask_next = n > 0 ? L.mask[0] : 0u;  // read one splat ahead
     for (int j = 0; j < n; ++j) {
-      const uint32_t reach = __builtin_amdgcn_readfirstlane(L.mask[j]);
+      const uint32_t reach = __builtin_amdgcn_readfirstlane(mask_next);
+      if (j + 1 < n) mask_next = L.mask[j + 1];
       if (reach == 0u) continue;
       uint32_t contributor;
       if constexpr (CELLS) {
