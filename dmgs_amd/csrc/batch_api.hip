@@ -94,14 +94,15 @@ struct StateLayout {
 
 // Per-instance lists (tile keys and entry ids, ping-pong) and their scratch.
 struct ListLayout {
-  size_t tkeys_a, tkeys_b, vals_a, vals_b, scratch, total;
+  size_t tkeys_a, tkeys_b, vals_a, vals_b, scratch, chunk, total;
   explicit ListLayout(size_t K) {
     size_t o = 0;
     tkeys_a = o; o += align_up(K * 4);
     tkeys_b = o; o += align_up(K * 4);
     vals_a = o; o += align_up(K * 4);
     vals_b = o; o += align_up(K * 4);
-    scratch = o; o += align_up(std::max(dmgs::radix_scratch_words(K), dmgs::duplicate_scratch_words(K)) * 4);
+    scratch = o; o += align_up(dmgs::radix_scratch_words(K) * 4);   // sort histograms
+    chunk = o; o += align_up(dmgs::duplicate_scratch_words(K) * 4);  // duplicate chunk starts
     total = o;
   }
 };
@@ -320,15 +321,22 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     d.values = at<uint32_t>(b->lists, c.L.vals_a);
     d.K = K;
     d.K_dev = c.count(b);
-    d.chunk_first = at<uint32_t>(b->lists, c.L.scratch);
+    d.chunk_first = at<uint32_t>(b->lists, c.L.chunk);
     d.view_major = b->cell_mode == 1;
+    // view-major lists: the duplicate also counts the first sort pass's digits
+    const int w0 = d.view_major ? dmgs::radix_min_passes_first_width(K, c.local_bits) : -1;
+    if (w0 > 0) {
+      d.hist = at<uint32_t>(b->lists, c.L.scratch);
+      d.hist_bits = w0;
+    }
     dmgs::launch_duplicate(d, st);
     uint32_t* kb = at<uint32_t>(b->lists, c.L.tkeys_b);
     uint32_t* vb = at<uint32_t>(b->lists, c.L.vals_b);
     if (d.view_major) {  // sort the cell-within-view bits only; the view order is already there
       const bool in_b = dmgs::radix_sort_pairs_min_passes(d.tile_keys, d.values, kb, vb, K,
                                                           c.local_bits,
-                                                          at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev);
+                                                          at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev,
+                                                          w0 > 0);
       if (in_b != c.lists_in_b()) return bfail(DMGS_EHIP, "internal: sorted list buffer mismatch");
       dmgs::launch_view_cell_ranges(in_b ? kb : d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells,
                                     a.views, st, d.K_dev);

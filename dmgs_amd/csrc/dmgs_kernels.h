@@ -48,9 +48,14 @@ size_t radix_scratch_words(size_t n);
 void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
                       uint32_t* scratch, hipStream_t s, const uint32_t* n_dev = nullptr);
 // The same with as few passes as the bits need (one pass up to 8 bits);
-// returns true when the result is in (k1, v1).
+// returns true when the result is in (k1, v1).  first_hist_ready: the first
+// pass's block histogram is already in the scratch (see DupArgs::hist).
 bool radix_sort_pairs_min_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
-                                 uint32_t* scratch, hipStream_t s, const uint32_t* n_dev = nullptr);
+                                 uint32_t* scratch, hipStream_t s, const uint32_t* n_dev = nullptr,
+                                 bool first_hist_ready = false);
+// Digit width of the first pass of radix_sort_pairs_min_passes, or -1 when
+// its blocks are not 4096 keys (the duplicate's chunks) for this n.
+int radix_min_passes_first_width(size_t n, int bits);
 
 // tiles_touched gathered into depth order, ready for the scan.
 void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, int n, hipStream_t s);
@@ -75,6 +80,10 @@ struct DupArgs {
   const uint32_t* K_dev;        // optional device count (clamped to K)
   uint32_t* chunk_first;        // duplicate_scratch_words(K) words
   bool view_major;
+  // optional: the first radix pass's histogram (digits = the low hist_bits of
+  // a key), written per 4096-slot chunk into the sort scratch
+  uint32_t* hist;
+  int hist_bits;
 };
 size_t duplicate_scratch_words(size_t K);
 void launch_duplicate(const DupArgs& a, hipStream_t s);

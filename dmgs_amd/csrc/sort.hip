@@ -341,7 +341,8 @@ size_t radix_scratch_words(size_t n) {
 
 template <int ITEMS>
 static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
-                         uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes) {
+                         uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes,
+                         bool first_hist_ready = false) {
   constexpr size_t kTile = (size_t)kSortThreads * ITEMS;
   const uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
   int widths[4];
@@ -357,8 +358,9 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
   int shift = 0;
   for (int p = 0; p < passes; ++p) {
     const int w = widths[p];
-    hipLaunchKernelGGL(radix_hist_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka, n, shift,
-                       w, hist, nb, n_dev);
+    if (!(p == 0 && first_hist_ready))
+      hipLaunchKernelGGL(radix_hist_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka, n, shift,
+                         w, hist, nb, n_dev);
     hipLaunchKernelGGL(radix_rowscan_kernel, dim3(1u << w), dim3(kScanThreads), 0, s, hist, nb, totals);
     hipLaunchKernelGGL(radix_scatter_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka,
                        (const uint32_t*)va, kb, vb, n, shift, w, (const uint32_t*)hist, (const uint32_t*)totals, nb,
@@ -374,11 +376,21 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
 }
 
 static void radix_dispatch(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
-                           uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes) {
+                           uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes,
+                           bool first_hist_ready = false) {
   if (sort_items(n) == 16)
-    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes);
+    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready);
   else
-    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes);
+    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready);
+}
+
+static int min_passes(int bits) { return bits <= 8 ? 1 : (bits <= 16 ? 2 : 4); }
+
+int radix_min_passes_first_width(size_t n, int bits) {
+  if (sort_items(n) != 16) return -1;  // blocks are not the duplicate's 4096-slot chunks
+  bits = max(0, min(32, bits));
+  const int passes = min_passes(bits);
+  return (bits + passes - 1) / passes;
 }
 
 void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
@@ -390,12 +402,12 @@ void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, si
 }
 
 bool radix_sort_pairs_min_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
-                                 uint32_t* scratch, hipStream_t s, const uint32_t* n_dev) {
+                                 uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, bool first_hist_ready) {
   if (n == 0) return false;
   if (bits < 0) bits = 0;
   if (bits > 32) bits = 32;
-  const int passes = bits <= 8 ? 1 : (bits <= 16 ? 2 : 4);
-  radix_dispatch(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes);
+  const int passes = min_passes(bits);
+  radix_dispatch(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready);
   return (passes & 1) != 0;
 }
 
@@ -443,9 +455,15 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
   __shared__ uint32_t s_rx[256], s_ry[256];  // multi-view: tile rectangle, [lo, hi) packed 16:16
   __shared__ int s_vcx[kDupViews];           // multi-view: cells per row of view v
   __shared__ uint32_t s_vtb[kDupViews];      // multi-view: key base of view v
+  __shared__ uint32_t s_hist[256];           // digit counts of this chunk (first sort pass)
   const uint32_t K = device_count(a.K, a.K_dev);
   const uint32_t lo_slot = blockIdx.x * kDupChunk;
-  if (lo_slot >= K) return;  // uniform per block
+  const uint32_t hmask = (1u << a.hist_bits) - 1u;
+  if (lo_slot >= K) {  // uniform per block: an empty chunk counts nothing
+    if (a.hist && threadIdx.x <= hmask) a.hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = 0u;
+    return;
+  }
+  if (a.hist) s_hist[threadIdx.x] = 0u;
   const uint32_t hi_slot = min(K, lo_slot + kDupChunk);
   const uint32_t P = (uint32_t)a.P;
   const uint32_t nviews = a.views ? (P + a.per_view - 1) / a.per_view : 0u;
@@ -531,6 +549,7 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
       }
       a.tile_keys[o] = key;
       a.values[o] = s_g[lo];
+      if (a.hist) atomicAdd(&s_hist[key & hmask], 1u);
     }
     const bool more = win_end < hi_slot && first + 256 < P;
     win_begin = s_end[255];  // the next window starts where this one ends
@@ -538,6 +557,9 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
     if (!more) break;
     first += 256;
   }
+  // the chunk is the first sort pass's block: its digit histogram, in the
+  // radix histogram's layout (digit-major over blocks)
+  if (a.hist && threadIdx.x <= hmask) a.hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s_hist[threadIdx.x];
 }
 
 size_t duplicate_scratch_words(size_t K) { return (K + kDupChunk - 1) / kDupChunk + 64; }
