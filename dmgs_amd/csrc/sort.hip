@@ -528,28 +528,65 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
     __syncthreads();
     const uint32_t win_end = min(s_end[255], hi_slot);
     const uint32_t o_begin = max(lo_slot, win_begin);
-    for (uint32_t o = o_begin + threadIdx.x; o < win_end; o += 256) {
-      int lo = 0, hi = 255;  // smallest j with s_end[j] > o
+    // Four consecutive slots per thread: one binary search per group, then
+    // the slots walk the entry's cells in order (a new entry starts at its
+    // first cell), and a full group leaves as one 16-byte store per array.
+    for (uint32_t o4 = (o_begin & ~3u) + 4 * threadIdx.x; o4 < win_end; o4 += 1024) {
+      const uint32_t o0 = max(o4, o_begin);
+      int lo = 0, hi = 255;  // smallest j with s_end[j] > o0
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (s_end[mid] > o) hi = mid;
+        if (s_end[mid] > o0) hi = mid;
         else lo = mid + 1;
       }
       const uint32_t beg = (lo == 0) ? win_begin : s_end[lo - 1];
-      const uint32_t local = o - beg;
-      const uint32_t wj = (uint32_t)s_w[lo];
-      const uint32_t row = local / wj;
-      const uint32_t cx = (uint32_t)s_x0[lo] + (local - row * wj), cy = (uint32_t)s_y0[lo] + row;
-      uint32_t key = s_tb[lo] + cy * (uint32_t)s_gx[lo] + cx;
-      if (a.views) {  // cell id | tile mask << 16
-        const uint32_t rx = s_rx[lo], ry = s_ry[lo];
-        const int2 rmin = make_int2((int)(rx & 0xFFFFu), (int)(ry & 0xFFFFu));
-        const int2 rmax = make_int2((int)(rx >> 16), (int)(ry >> 16));
-        key |= cell_mask(rmin, rmax, (int)cx, (int)cy) << DMGS_CELL_KEY_BITS;
+      uint32_t end = s_end[lo], wj = (uint32_t)s_w[lo];
+      uint32_t row = (o0 - beg) / wj;
+      uint32_t col = (o0 - beg) - row * wj;
+      uint32_t kk[4], vv[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t o = o4 + e;
+        kk[e] = vv[e] = 0u;
+        if (o < o0 || o >= win_end) continue;
+        if (o > o0) {  // next slot: next cell of this entry, or the first cell of the next non-empty one
+          if (o >= end) {
+            do {
+              ++lo;
+            } while (s_end[lo] <= o);
+            end = s_end[lo];
+            wj = (uint32_t)s_w[lo];
+            row = col = 0;
+          } else if (++col == wj) {
+            col = 0;
+            ++row;
+          }
+        }
+        const uint32_t cx = (uint32_t)s_x0[lo] + col, cy = (uint32_t)s_y0[lo] + row;
+        uint32_t key = s_tb[lo] + cy * (uint32_t)s_gx[lo] + cx;
+        if (a.views) {  // cell id | tile mask << 16
+          const uint32_t rx = s_rx[lo], ry = s_ry[lo];
+          const int2 rmin = make_int2((int)(rx & 0xFFFFu), (int)(ry & 0xFFFFu));
+          const int2 rmax = make_int2((int)(rx >> 16), (int)(ry >> 16));
+          key |= cell_mask(rmin, rmax, (int)cx, (int)cy) << DMGS_CELL_KEY_BITS;
+        }
+        kk[e] = key;
+        vv[e] = s_g[lo];
+        if (a.hist) atomicAdd(&s_hist[key & hmask], 1u);
       }
-      a.tile_keys[o] = key;
-      a.values[o] = s_g[lo];
-      if (a.hist) atomicAdd(&s_hist[key & hmask], 1u);
+      if (o4 >= o_begin && o4 + 4 <= win_end) {
+        *reinterpret_cast<uint4*>(a.tile_keys + o4) = make_uint4(kk[0], kk[1], kk[2], kk[3]);
+        *reinterpret_cast<uint4*>(a.values + o4) = make_uint4(vv[0], vv[1], vv[2], vv[3]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t o = o4 + e;
+          if (o >= o_begin && o < win_end) {
+            a.tile_keys[o] = kk[e];
+            a.values[o] = vv[e];
+          }
+        }
+      }
     }
     const bool more = win_end < hi_slot && first + 256 < P;
     win_begin = s_end[255];  // the next window starts where this one ends
