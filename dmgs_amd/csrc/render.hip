@@ -638,7 +638,8 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   for (int k = 0; k < 4; ++k) pyf[k] = (float)(c.py + 4 * k);
 
   float T[4], T_final[4], g0[4], g1[4], g2[4], bg_term[4];
-  float acc0[4], acc1[4], acc2[4], last_alpha[4], lc0[4], lc1[4], lc2[4];
+  // pc = last_alpha * last colour: the recursion's next term, formed once
+  float acc0[4], acc1[4], acc2[4], last_alpha[4], pc0[4], pc1[4], pc2[4];
   uint32_t last_contributor[4];
   uint32_t m = 0;
 #pragma unroll
@@ -661,7 +662,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
     bg_term[k] = -T_final[k] * (c.bg0 * g0[k] + c.bg1 * g1[k] + c.bg2 * g2[k]);
     T[k] = T_final[k];
     acc0[k] = acc1[k] = acc2[k] = 0.f;
-    last_alpha[k] = lc0[k] = lc1[k] = lc2[k] = 0.f;
+    last_alpha[k] = pc0[k] = pc1[k] = pc2[k] = 0.f;
     m = max(m, last_contributor[k]);
   }
   for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
@@ -734,12 +735,13 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
           const float inv_one_m = __builtin_amdgcn_rcpf(1.f - alpha);
           T[k] = T[k] * inv_one_m;
           const float dchannel_dcolor = alpha * T[k];
-          acc0[k] = last_alpha[k] * lc0[k] + (1.f - last_alpha[k]) * acc0[k];
-          acc1[k] = last_alpha[k] * lc1[k] + (1.f - last_alpha[k]) * acc1[k];
-          acc2[k] = last_alpha[k] * lc2[k] + (1.f - last_alpha[k]) * acc2[k];
-          lc0[k] = col.x;
-          lc1[k] = col.y;
-          lc2[k] = col.z;
+          const float om = 1.f - last_alpha[k];
+          acc0[k] = fmaf(om, acc0[k], pc0[k]);
+          acc1[k] = fmaf(om, acc1[k], pc1[k]);
+          acc2[k] = fmaf(om, acc2[k], pc2[k]);
+          pc0[k] = alpha * col.x;
+          pc1[k] = alpha * col.y;
+          pc2[k] = alpha * col.z;
           float dL_dalpha = (col.x - acc0[k]) * g0[k] + (col.y - acc1[k]) * g1[k] + (col.z - acc2[k]) * g2[k];
           v[6] += dchannel_dcolor * g0[k];
           v[7] += dchannel_dcolor * g1[k];
@@ -747,16 +749,16 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
           dL_dalpha *= T[k];
           last_alpha[k] = alpha;
           dL_dalpha += bg_term[k] * inv_one_m;
-          const float dL_dG = co.w * dL_dalpha;
-          const float gdx = G * dx, gdy = G * dy;
-          const float dG_ddelx = -gdx * co.x - gdy * co.y;
-          const float dG_ddely = -gdy * co.z - gdx * co.y;
-          // constant factors (W/2, H/2, -1/2) are applied once per splat at the flush
-          v[0] += dL_dG * dG_ddelx;
-          v[1] += dL_dG * dG_ddely;
-          v[2] += gdx * dx * dL_dG;
-          v[3] += gdx * dy * dL_dG;
-          v[4] += gdy * dy * dL_dG;
+          // with t = dL/dG * G: dG/d(dx) = -G (a dx + b dy), dG/d(dy) = -G (b dx + c dy),
+          // dG/d(a, b, c) = G (dx^2, dx dy, dy^2) times the constant factors
+          // (W/2, H/2, -1/2) applied once per splat at the flush
+          const float t = co.w * dL_dalpha * G;
+          const float tdx = t * dx, tdy = t * dy;
+          v[0] = fmaf(-tdy, co.y, fmaf(-tdx, co.x, v[0]));
+          v[1] = fmaf(-tdx, co.y, fmaf(-tdy, co.z, v[1]));
+          v[2] = fmaf(tdx, dx, v[2]);
+          v[3] = fmaf(tdx, dy, v[3]);
+          v[4] = fmaf(tdy, dy, v[4]);
           v[5] += G * dL_dalpha;
         }
       }
