@@ -70,7 +70,7 @@ int bits_for(uint32_t n) {
 // scan, the depth-sort scratch, per-tile ranges and per-pixel blend state.
 struct StateLayout {
   size_t splats, keys_a, keys_b, order_a, order_b, rects, rects_sorted, radii, offsets, gscratch, wave_counts,
-      status, ranges, final_T, n_contrib, total;
+      status, tile_order, ranges, final_T, n_contrib, total;
   StateLayout(size_t N, size_t T, size_t X, size_t wave_words) {
     size_t o = 0;
     splats = o; o += align_up(N * DMGS_SPLAT_F4 * 16);
@@ -85,6 +85,7 @@ struct StateLayout {
     gscratch = o; o += align_up(std::max(dmgs::radix_scratch_words(N), dmgs::scan_scratch_words(N)) * 4);
     wave_counts = o; o += align_up(wave_words * 4);
     status = o; o += align_up(4);
+    tile_order = o; o += align_up(T * 4);
     ranges = o; o += align_up(T * 8);
     final_T = o; o += align_up(X * 4);
     n_contrib = o; o += align_up(X * 4);
@@ -181,6 +182,7 @@ dmgs::RenderArgs render_args(const dmgs_batch* b, const Ctx& c) {
   r.final_T = at<float>(b->state, c.S.final_T);
   r.n_contrib = at<uint32_t>(b->state, c.S.n_contrib);
   r.gacc = static_cast<float*>(b->bwd_ws);
+  r.tile_order = at<uint32_t>(b->state, c.S.tile_order);
   return r;
 }
 
@@ -346,6 +348,8 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
       dmgs::launch_tile_ranges(d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells, st, d.K_dev,
                                DMGS_CELL_KEY_MASK);
     }
+    // blend dispatch order: tiles with the longest lists first
+    dmgs::launch_tile_order(render_args(b, c), at<uint32_t>(b->state, c.S.tile_order), st);
     if ((rc = bcheck("batch binning", st))) return rc;
   }
   if (stages & DMGS_STAGE_BLEND) {

@@ -157,6 +157,8 @@ struct RenderArgs {
   // backward only
   const float* dL_dpix;  // (3, H, W)
   float* gacc;           // (P, DMGS_GACC_STRIDE)
+  // optional (multi-view): work-group slot -> tile, heaviest tiles first
+  const uint32_t* tile_order;
 };
 // Work-group shape / LDS batch depth of the blend kernels (results are the
 // same for every variant; the autotuner picks the fastest).  The strip
@@ -173,6 +175,9 @@ enum RenderVariant {
   kRenderVariants = 7
 };
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
+// Multi-view: order the a.num_tiles tiles by decreasing list length (a
+// binned estimate of their cost) so the longest start first; one work-group.
+void launch_tile_order(const RenderArgs& a, uint32_t* order, hipStream_t s);
 void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
 
 // ---- geometry backward ----

@@ -169,8 +169,13 @@ template <int SUB>
 __device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
   constexpr int Q = DMGS_BLOCK_X / SUB;  // workgroups per tile edge
   TileCtx c;
-  c.tile = blockIdx.x / (Q * Q);
-  const int q = blockIdx.x - c.tile * Q * Q;
+  const int slot = blockIdx.x / (Q * Q);
+  const int q = blockIdx.x - slot * Q * Q;
+  c.tile = slot;
+  if (a.tile_order) {  // a stale or unset order entry falls back to the identity
+    const uint32_t t = a.tile_order[slot];
+    if (t < (uint32_t)a.num_tiles) c.tile = (int)t;
+  }
   int lt = c.tile, gx = a.grid_x;
   int64_t pbase = 0;
   int list = c.tile;
@@ -806,6 +811,48 @@ static void launch_bwd_strips_t(const RenderArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((render_bwd_strips_kernel<BATCH, true>), dim3(a.num_tiles), dim3(64), 0, s, a);
   else
     hipLaunchKernelGGL((render_bwd_strips_kernel<BATCH, false>), dim3(a.num_tiles), dim3(64), 0, s, a);
+}
+
+// Longest-first scheduling of the blend: a tile's cost is about the length
+// of its binning cell's list, so tiles are bucketed by log2 of that length
+// (8 buckets per octave, 256 in all) and emitted heaviest bucket first.  Only
+// the dispatch order changes; every tile's result is the same in any order.
+__device__ __forceinline__ uint32_t tile_cost_bucket(const RenderArgs& a, int t, uint32_t K) {
+  const ViewDesc& v = a.views[a.tile_view[t]];
+  const int lt = t - v.tile_base, gx = v.grid_x;
+  const int ltx = lt % gx, lty = lt / gx;
+  const int cell = v.cell_base + (lty >> DMGS_CELL_SHIFT) * cells_x(gx) + (ltx >> DMGS_CELL_SHIFT);
+  const uint2 r = a.ranges[cell];
+  const uint32_t hi = min(r.y, K), lo = min(r.x, hi);
+  const uint32_t len = hi - lo;
+  if (len == 0) return 255u;
+  return 254u - min(254u, (uint32_t)(8.0f * __log2f((float)len)));
+}
+
+__global__ __launch_bounds__(1024) void tile_order_kernel(RenderArgs a, uint32_t* __restrict__ order) {
+  __shared__ uint32_t cnt[256];
+  uint32_t K = a.K;
+  if (a.K_dev) K = min(K, *a.K_dev);
+  if (threadIdx.x < 256) cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  for (int t = threadIdx.x; t < a.num_tiles; t += blockDim.x) atomicAdd(&cnt[tile_cost_bucket(a, t, K)], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int b = 0; b < 256; ++b) {
+      const uint32_t c = cnt[b];
+      cnt[b] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < a.num_tiles; t += blockDim.x)
+    order[atomicAdd(&cnt[tile_cost_bucket(a, t, K)], 1u)] = (uint32_t)t;
+}
+
+void launch_tile_order(const RenderArgs& a, uint32_t* order, hipStream_t s) {
+  if (a.num_tiles <= 0 || !a.views) return;
+  hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(1024), 0, s, a, order);
 }
 
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {
