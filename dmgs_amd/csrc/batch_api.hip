@@ -70,7 +70,7 @@ int bits_for(uint32_t n) {
 // scan, the depth-sort scratch, per-tile ranges and per-pixel blend state.
 struct StateLayout {
   size_t splats, keys_a, keys_b, order_a, order_b, rects, rects_sorted, radii, offsets, gscratch, wave_counts,
-      status, tile_order, ranges, final_T, n_contrib, total;
+      status, tile_order, tile_bucket, bucket_hist, ranges, final_T, n_contrib, total;
   StateLayout(size_t N, size_t T, size_t X, size_t wave_words) {
     size_t o = 0;
     splats = o; o += align_up(N * DMGS_SPLAT_F4 * 16);
@@ -86,6 +86,8 @@ struct StateLayout {
     wave_counts = o; o += align_up(wave_words * 4);
     status = o; o += align_up(4);
     tile_order = o; o += align_up(T * 4);
+    tile_bucket = o; o += align_up(T);
+    bucket_hist = o; o += align_up(512 * 4);
     ranges = o; o += align_up(T * 8);
     final_T = o; o += align_up(X * 4);
     n_contrib = o; o += align_up(X * 4);
@@ -349,7 +351,8 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
                                DMGS_CELL_KEY_MASK);
     }
     // blend dispatch order: tiles with the longest lists first
-    dmgs::launch_tile_order(render_args(b, c), at<uint32_t>(b->state, c.S.tile_order), st);
+    dmgs::launch_tile_order(render_args(b, c), at<uint32_t>(b->state, c.S.tile_order),
+                            at<uint8_t>(b->state, c.S.tile_bucket), at<uint32_t>(b->state, c.S.bucket_hist), st);
     if ((rc = bcheck("batch binning", st))) return rc;
   }
   if (stages & DMGS_STAGE_BLEND) {

@@ -153,6 +153,24 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   return v;
 }
 
+// Exclusive scan of one value per thread over the work-group (s_wave: one
+// word per wave); total receives the sum.  Every thread must call it.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t inc = wave_inclusive_scan(v);
+  if (lane == 63) s_wave[wave] = inc;
+  __syncthreads();
+  uint32_t wave_off = 0, tot = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    const uint32_t x = s_wave[w];
+    if (w < wave) wave_off += x;
+    tot += x;
+  }
+  total = tot;
+  __syncthreads();
+  return wave_off + inc - v;
+}
+
 // Load the first 16 SH coefficients (RGB each) of one Gaussian into registers;
 // 16-byte vector loads when the per-Gaussian stride allows it.
 __device__ __forceinline__ void load_sh16(const float* __restrict__ sh, int M, float (&r)[48]) {

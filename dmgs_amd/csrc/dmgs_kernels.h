@@ -176,8 +176,9 @@ enum RenderVariant {
 };
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
 // Multi-view: order the a.num_tiles tiles by decreasing list length (a
-// binned estimate of their cost) so the longest start first; one work-group.
-void launch_tile_order(const RenderArgs& a, uint32_t* order, hipStream_t s);
+// binned estimate of their cost) so the longest start first.  bucket: one
+// byte per tile, hist: 512 words of scratch.
+void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s);
 void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
 
 // ---- geometry backward ----

@@ -829,30 +829,55 @@ __device__ __forceinline__ uint32_t tile_cost_bucket(const RenderArgs& a, int t,
   return 254u - min(254u, (uint32_t)(8.0f * __log2f((float)len)));
 }
 
-__global__ __launch_bounds__(1024) void tile_order_kernel(RenderArgs a, uint32_t* __restrict__ order) {
+// Pass 1 (all tiles in parallel): bucket per tile, global bucket histogram.
+__global__ __launch_bounds__(256) void tile_bucket_kernel(RenderArgs a, uint8_t* __restrict__ bucket,
+                                                          uint32_t* __restrict__ hist) {
   __shared__ uint32_t cnt[256];
   uint32_t K = a.K;
   if (a.K_dev) K = min(K, *a.K_dev);
-  if (threadIdx.x < 256) cnt[threadIdx.x] = 0u;
+  cnt[threadIdx.x] = 0u;
   __syncthreads();
-  for (int t = threadIdx.x; t < a.num_tiles; t += blockDim.x) atomicAdd(&cnt[tile_cost_bucket(a, t, K)], 1u);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int b = 0; b < 256; ++b) {
-      const uint32_t c = cnt[b];
-      cnt[b] = run;
-      run += c;
-    }
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t < a.num_tiles) {
+    const uint32_t b = tile_cost_bucket(a, t, K);
+    bucket[t] = (uint8_t)b;
+    atomicAdd(&cnt[b], 1u);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < a.num_tiles; t += blockDim.x)
-    order[atomicAdd(&cnt[tile_cost_bucket(a, t, K)], 1u)] = (uint32_t)t;
+  if (cnt[threadIdx.x]) atomicAdd(&hist[threadIdx.x], cnt[threadIdx.x]);
 }
 
-void launch_tile_order(const RenderArgs& a, uint32_t* order, hipStream_t s) {
+// Pass 2 (all tiles in parallel): every work-group scans the 256-bucket
+// histogram (hist[0..255]), reserves a run per bucket for its tiles from the
+// cursors (hist[256..511], zero on entry) and writes its tiles there.
+__global__ __launch_bounds__(256) void tile_order_kernel(const uint8_t* __restrict__ bucket, uint32_t* __restrict__ hist,
+                                                         int num_tiles, uint32_t* __restrict__ order) {
+  __shared__ uint32_t start[256], cnt[256], base[256];
+  __shared__ uint32_t s_wave[4];
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t h = hist[threadIdx.x];
+  uint32_t total;
+  start[threadIdx.x] = block_exclusive_scan(h, s_wave, total);
+  cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  uint32_t b = 0, rank = 0;
+  if (t < num_tiles) {
+    b = bucket[t];
+    rank = atomicAdd(&cnt[b], 1u);
+  }
+  __syncthreads();
+  if (cnt[threadIdx.x]) base[threadIdx.x] = start[threadIdx.x] + atomicAdd(&hist[256 + threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (t < num_tiles) order[base[b] + rank] = (uint32_t)t;
+}
+
+void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s) {
   if (a.num_tiles <= 0 || !a.views) return;
-  hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(1024), 0, s, a, order);
+  const unsigned blocks = (unsigned)((a.num_tiles + 255) / 256);
+  launch_fill_u32(hist, 512, 0u, s);
+  hipLaunchKernelGGL(tile_bucket_kernel, dim3(blocks), dim3(256), 0, s, a, bucket, hist);
+  hipLaunchKernelGGL(tile_order_kernel, dim3(blocks), dim3(256), 0, s, (const uint8_t*)bucket, hist, a.num_tiles,
+                     order);
 }
 
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {

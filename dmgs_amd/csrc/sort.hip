@@ -29,21 +29,6 @@ constexpr int kSortWaves = kSortThreads / 64;
 // ---------------------------------------------------------------------------
 // block-wide helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t inc = wave_inclusive_scan(v);
-  if (lane == 63) s_wave[wave] = inc;
-  __syncthreads();
-  uint32_t wave_off = 0, tot = 0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-    const uint32_t x = s_wave[w];
-    if (w < wave) wave_off += x;
-    tot += x;
-  }
-  total = tot;
-  __syncthreads();
-  return wave_off + inc - v;
-}
 
 // ---------------------------------------------------------------------------
 // scan
