@@ -70,7 +70,7 @@ int bits_for(uint32_t n) {
 // scan, the depth-sort scratch, per-tile ranges and per-pixel blend state.
 struct StateLayout {
   size_t splats, keys_a, keys_b, order_a, order_b, rects, rects_sorted, radii, offsets, gscratch, wave_counts,
-      status, tile_order, tile_bucket, bucket_hist, ranges, final_T, n_contrib, total;
+      status, tile_order, tile_bucket, bucket_hist, tile_work, ranges, final_T, n_contrib, total;
   StateLayout(size_t N, size_t T, size_t X, size_t wave_words) {
     size_t o = 0;
     splats = o; o += align_up(N * DMGS_SPLAT_F4 * 16);
@@ -88,6 +88,7 @@ struct StateLayout {
     tile_order = o; o += align_up(T * 4);
     tile_bucket = o; o += align_up(T);
     bucket_hist = o; o += align_up(512 * 4);
+    tile_work = o; o += align_up(T * 4);
     ranges = o; o += align_up(T * 8);
     final_T = o; o += align_up(X * 4);
     n_contrib = o; o += align_up(X * 4);
@@ -185,6 +186,9 @@ dmgs::RenderArgs render_args(const dmgs_batch* b, const Ctx& c) {
   r.n_contrib = at<uint32_t>(b->state, c.S.n_contrib);
   r.gacc = static_cast<float*>(b->bwd_ws);
   r.tile_order = at<uint32_t>(b->state, c.S.tile_order);
+  // the strip forward measures each tile's backward list length
+  const bool strips_fwd = b->variant == dmgs::kRenderStrips64 || b->variant == dmgs::kRenderStrips128;
+  r.tile_work = strips_fwd ? at<uint32_t>(b->state, c.S.tile_work) : nullptr;
   return r;
 }
 
@@ -380,6 +384,9 @@ int dmgs_batched_backward(const dmgs_batch* b, const float* params, const float*
   if (stages & DMGS_STAGE_BLEND) {  // accumulates into bwd_ws, which holds zeros on entry
     dmgs::RenderArgs r = render_args(b, c);
     r.dL_dpix = dL_dimages;
+    // re-order the tiles by the list lengths the strip forward measured
+    dmgs::launch_tile_order_bwd(r, at<uint32_t>(b->state, c.S.tile_order), at<uint8_t>(b->state, c.S.tile_bucket),
+                                at<uint32_t>(b->state, c.S.bucket_hist), st);
     dmgs::launch_render_bwd(r, st, b->variant);
     if ((rc = bcheck("batch render backward", st))) return rc;
   }

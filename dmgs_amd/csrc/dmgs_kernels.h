@@ -159,6 +159,9 @@ struct RenderArgs {
   float* gacc;           // (P, DMGS_GACC_STRIDE)
   // optional (multi-view): work-group slot -> tile, heaviest tiles first
   const uint32_t* tile_order;
+  // optional (multi-view, strip forward): per tile, the largest last
+  // contributor of its pixels (the backward's list length)
+  uint32_t* tile_work;
 };
 // Work-group shape / LDS batch depth of the blend kernels (results are the
 // same for every variant; the autotuner picks the fastest).  The strip
@@ -179,6 +182,9 @@ void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant = kRender
 // binned estimate of their cost) so the longest start first.  bucket: one
 // byte per tile, hist: 512 words of scratch.
 void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s);
+// The same with a.tile_work (written by the strip forward) as each tile's
+// cost: the backward's exact list length.
+void launch_tile_order_bwd(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s);
 void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
 
 // ---- geometry backward ----

@@ -467,6 +467,11 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
       }
     }
   }
+  if (a.tile_work) {  // the backward walks list positions below the tile's largest last contributor
+    uint32_t m = max(max(last_contributor[0], last_contributor[1]), max(last_contributor[2], last_contributor[3]));
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+    if (threadIdx.x == 0) a.tile_work[c.tile] = m;
+  }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (!(c.px < c.W && c.py + 4 * k < c.H)) continue;
@@ -817,21 +822,22 @@ static void launch_bwd_strips_t(const RenderArgs& a, hipStream_t s) {
 // of its binning cell's list, so tiles are bucketed by log2 of that length
 // (8 buckets per octave, 256 in all) and emitted heaviest bucket first.  Only
 // the dispatch order changes; every tile's result is the same in any order.
-__device__ __forceinline__ uint32_t tile_cost_bucket(const RenderArgs& a, int t, uint32_t K) {
+__device__ __forceinline__ uint32_t tile_cost_bucket(const RenderArgs& a, int t, uint32_t K, bool use_work) {
   const ViewDesc& v = a.views[a.tile_view[t]];
   const int lt = t - v.tile_base, gx = v.grid_x;
   const int ltx = lt % gx, lty = lt / gx;
   const int cell = v.cell_base + (lty >> DMGS_CELL_SHIFT) * cells_x(gx) + (ltx >> DMGS_CELL_SHIFT);
   const uint2 r = a.ranges[cell];
   const uint32_t hi = min(r.y, K), lo = min(r.x, hi);
-  const uint32_t len = hi - lo;
+  uint32_t len = hi - lo;
+  if (use_work) len = min(len, a.tile_work[t]);
   if (len == 0) return 255u;
   return 254u - min(254u, (uint32_t)(8.0f * __log2f((float)len)));
 }
 
 // Pass 1 (all tiles in parallel): bucket per tile, global bucket histogram.
 __global__ __launch_bounds__(256) void tile_bucket_kernel(RenderArgs a, uint8_t* __restrict__ bucket,
-                                                          uint32_t* __restrict__ hist) {
+                                                          uint32_t* __restrict__ hist, int use_work) {
   __shared__ uint32_t cnt[256];
   uint32_t K = a.K;
   if (a.K_dev) K = min(K, *a.K_dev);
@@ -839,7 +845,7 @@ __global__ __launch_bounds__(256) void tile_bucket_kernel(RenderArgs a, uint8_t*
   __syncthreads();
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t < a.num_tiles) {
-    const uint32_t b = tile_cost_bucket(a, t, K);
+    const uint32_t b = tile_cost_bucket(a, t, K, use_work != 0);
     bucket[t] = (uint8_t)b;
     atomicAdd(&cnt[b], 1u);
   }
@@ -871,13 +877,22 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const uint8_t* __restri
   if (t < num_tiles) order[base[b] + rank] = (uint32_t)t;
 }
 
-void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s) {
+static void tile_order_impl(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, bool use_work,
+                            hipStream_t s) {
   if (a.num_tiles <= 0 || !a.views) return;
   const unsigned blocks = (unsigned)((a.num_tiles + 255) / 256);
   launch_fill_u32(hist, 512, 0u, s);
-  hipLaunchKernelGGL(tile_bucket_kernel, dim3(blocks), dim3(256), 0, s, a, bucket, hist);
+  hipLaunchKernelGGL(tile_bucket_kernel, dim3(blocks), dim3(256), 0, s, a, bucket, hist, use_work ? 1 : 0);
   hipLaunchKernelGGL(tile_order_kernel, dim3(blocks), dim3(256), 0, s, (const uint8_t*)bucket, hist, a.num_tiles,
                      order);
+}
+
+void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s) {
+  tile_order_impl(a, order, bucket, hist, false, s);
+}
+
+void launch_tile_order_bwd(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s) {
+  if (a.tile_work) tile_order_impl(a, order, bucket, hist, true, s);
 }
 
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {
