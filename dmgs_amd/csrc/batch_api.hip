@@ -309,8 +309,8 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     uint32_t* status = at<uint32_t>(b->state, c.S.status);
     dmgs::launch_fill_u32(status, 1, 0u, st);
     dmgs::launch_mv_compact(a, a.depth_keys, keys, a.order, b->cell_mode == 1, status, st);
-    dmgs::radix_sort_pairs(keys, a.order, at<uint32_t>(b->state, c.S.keys_b), at<uint32_t>(b->state, c.S.order_b),
-                           N, 32, gs, st, V_dev);
+    dmgs::radix_sort_depth_keys(keys, a.order, at<uint32_t>(b->state, c.S.keys_b),
+                                at<uint32_t>(b->state, c.S.order_b), N, gs, st, V_dev);
     dmgs::launch_gather_rects(a.rects, a.order, at<uint2>(b->state, c.S.rects_sorted), offs, N, st, V_dev);
     dmgs::launch_inclusive_scan(offs, offs, N, gs, st);
     if ((rc = bcheck("batch preprocess / depth sort / scan", st))) return rc;

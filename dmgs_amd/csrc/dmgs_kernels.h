@@ -53,6 +53,10 @@ void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, si
 bool radix_sort_pairs_min_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
                                  uint32_t* scratch, hipStream_t s, const uint32_t* n_dev = nullptr,
                                  bool first_hist_ready = false);
+// radix_sort_pairs over 32-bit float-depth keys (4 passes): the three low
+// bytes are close to uniform and are counted with plain LDS atomics.
+void radix_sort_depth_keys(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, uint32_t* scratch,
+                           hipStream_t s, const uint32_t* n_dev = nullptr);
 // Digit width of the first pass of radix_sort_pairs_min_passes, or -1 when
 // its blocks are not 4096 keys (the duplicate's chunks) for this n.
 int radix_min_passes_first_width(size_t n, int bits);

@@ -163,15 +163,23 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, int nbits, bool vali
   return peers;
 }
 
-template <int ITEMS>
+// SPREAD: the digits are close to uniform (the low bytes of depth keys), so
+// per-wave counters with one plain LDS atomic per key rarely collide;
+// otherwise lanes with equal digits are grouped by ballots first.
+template <int ITEMS, bool SPREAD>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, size_t n,
                                                                  int shift, int nbits,
                                                                  uint32_t* __restrict__ hist, uint32_t nb,
                                                                  const uint32_t* __restrict__ n_dev) {
   __shared__ uint32_t cnt[256];
+  __shared__ uint32_t wcnt[SPREAD ? kSortWaves : 1][256];
   if (n_dev) n = min(n, (size_t)*n_dev);  // device-side length (blocks past it count nothing)
   const uint32_t mask = (1u << nbits) - 1u;
   cnt[threadIdx.x] = 0;
+  if constexpr (SPREAD) {
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) wcnt[w][threadIdx.x] = 0;
+  }
   __syncthreads();
   const size_t base = (size_t)blockIdx.x * (kSortThreads * ITEMS);
   const int lane = threadIdx.x & 63;
@@ -193,12 +201,23 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
       const size_t idx = base + (size_t)i * kSortThreads + threadIdx.x;
       const bool valid = idx < n;
       const uint32_t d = (k[i] >> shift) & mask;
-      const uint64_t peers = digit_peers(d, nbits, valid);
-      if (valid && lane == 63 - __clzll(peers)) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
+      if constexpr (SPREAD) {
+        if (valid) atomicAdd(&wcnt[threadIdx.x >> 6][d], 1u);
+      } else {
+        const uint64_t peers = digit_peers(d, nbits, valid);
+        if (valid && lane == 63 - __clzll(peers)) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
+      }
     }
   }
   __syncthreads();
-  if (threadIdx.x <= mask) hist[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+  if (threadIdx.x <= mask) {
+    uint32_t c = cnt[threadIdx.x];
+    if constexpr (SPREAD) {
+#pragma unroll
+      for (int w = 0; w < kSortWaves; ++w) c += wcnt[w][threadIdx.x];
+    }
+    hist[(size_t)threadIdx.x * nb + blockIdx.x] = c;
+  }
 }
 
 // Exclusive scan of row `blockIdx.x` (one digit, nb block counts) in place;
@@ -327,7 +346,7 @@ size_t radix_scratch_words(size_t n) {
 template <int ITEMS>
 static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
                          uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes,
-                         bool first_hist_ready = false) {
+                         bool first_hist_ready = false, int spread_below = 0) {
   constexpr size_t kTile = (size_t)kSortThreads * ITEMS;
   const uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
   int widths[4];
@@ -343,9 +362,14 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
   int shift = 0;
   for (int p = 0; p < passes; ++p) {
     const int w = widths[p];
-    if (!(p == 0 && first_hist_ready))
-      hipLaunchKernelGGL(radix_hist_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka, n, shift,
-                         w, hist, nb, n_dev);
+    if (p == 0 && first_hist_ready) {
+    } else if (shift + w <= spread_below) {
+      hipLaunchKernelGGL((radix_hist_kernel<ITEMS, true>), dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka, n,
+                         shift, w, hist, nb, n_dev);
+    } else {
+      hipLaunchKernelGGL((radix_hist_kernel<ITEMS, false>), dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka,
+                         n, shift, w, hist, nb, n_dev);
+    }
     hipLaunchKernelGGL(radix_rowscan_kernel, dim3(1u << w), dim3(kScanThreads), 0, s, hist, nb, totals);
     hipLaunchKernelGGL(radix_scatter_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka,
                        (const uint32_t*)va, kb, vb, n, shift, w, (const uint32_t*)hist, (const uint32_t*)totals, nb,
@@ -362,11 +386,17 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
 
 static void radix_dispatch(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
                            uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes,
-                           bool first_hist_ready = false) {
+                           bool first_hist_ready = false, int spread_below = 0) {
   if (sort_items(n) == 16)
-    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready);
+    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready, spread_below);
   else
-    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready);
+    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready, spread_below);
+}
+
+void radix_sort_depth_keys(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, uint32_t* scratch,
+                           hipStream_t s, const uint32_t* n_dev) {
+  if (n == 0) return;
+  radix_dispatch(k0, v0, k1, v1, n, 32, scratch, s, n_dev, 4, false, 24);
 }
 
 static int min_passes(int bits) { return bits <= 8 ? 1 : (bits <= 16 ? 2 : 4); }
