@@ -68,6 +68,66 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _time_ms(fn, reps=20, warmup=3):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        t.append(a.elapsed_time(b))
+    return float(np.median(t))
+
+
+def binding_run(dev):
+    """Mesh -> Gaussian binding at the sizes of the reference's only
+    published timings (proto&effi_stage2.ipynb:74-76,161: 98 528 faces x 6 =
+    591 168 Gaussians, stage-2 forward pieces 1.25 + 0.28 + 0.21 + 1.25 ms on
+    an unstated NVIDIA GPU) and at the mesh-bound workload (icosphere of
+    81 920 faces, 6 Dirichlet-barycentric Gaussians per face, stage 3)."""
+    from dmgs_amd import binding as PB
+    from dmgs_amd import scenes as S
+    v7, f7 = S.icosphere(7)
+    faces = f7[:98528].to(dev)
+    verts = v7.to(dev).requires_grad_(True)
+    bc, rad = PB.barycentric_v2(6, device=dev)
+    sf = torch.tensor([0.4], device=dev, requires_grad=True)
+
+    def s2_fwd():
+        with torch.no_grad():
+            PB.bind_stage2(verts, faces, bc, sf, rad, spatial_lr_scale=2.5)
+
+    def s2_fwd_bwd():
+        xyz, cov, _ = PB.bind_stage2(verts, faces, bc, sf, rad, spatial_lr_scale=2.5)
+        torch.autograd.backward([xyz, cov], [gx, gc])
+
+    with torch.no_grad():
+        xyz0, cov0, _ = PB.bind_stage2(verts, faces, bc, sf, rad, spatial_lr_scale=2.5)
+    gx, gc = torch.ones_like(xyz0), torch.ones_like(cov0)
+    m = PB.mesh_scene(subdivisions=6, seed=0, device=dev).requires_grad_(True)
+
+    def s3_fwd():
+        with torch.no_grad():
+            m.renew_gaussian()
+
+    def s3_fwd_bwd():
+        m.renew_gaussian()
+        torch.autograd.backward([m._xyz, m._rot, m._scale], [torch.ones_like(m._xyz), torch.ones_like(m._rot),
+                                                             torch.ones_like(m._scale)])
+
+    return {"stage2_faces": int(faces.shape[0]), "stage2_gaussians": int(xyz0.shape[0]),
+            "stage2_fwd_ms": round(_time_ms(s2_fwd), 4), "stage2_fwd_bwd_ms": round(_time_ms(s2_fwd_bwd), 4),
+            "reference_stage2_fwd_ms": 2.99,
+            "reference_note": "sum of proto&effi_stage2.ipynb:74-76,161 (face frame + means, M, L, Sigma + "
+                              "expand; unstated NVIDIA GPU, PyTorch ops)",
+            "stage3_faces": int(m.faces.shape[0]), "stage3_gaussians": int(m.bc_coords.shape[0]),
+            "stage3_fwd_ms": round(_time_ms(s3_fwd), 4), "stage3_fwd_bwd_ms": round(_time_ms(s3_fwd_bwd), 4)}
+
+
 class StageTimer:
     """Device events around named stages, recorded on the current stream."""
 
@@ -414,6 +474,8 @@ def main():
         "step_algorithmic_GBs": round(step_bytes / (ms_step * 1e-3) / 1e9, 2),
     }
     log(f"batched {it_s:.2f} it/s; plain {plain['value'] if plain else 'skipped'}; stages {stage_ms}")
+    result["binding"] = binding_run(dev)
+    log(f"binding {result['binding']}")
     if args.serving_requests > 0:
         log(f"serving: {args.serving_requests} requests at {args.serving_rate}/s over 4 resident scenes")
         del step, loss, r
