@@ -30,10 +30,11 @@ def _mesh(sub, seed, G=6, per_gaussian=True):
     return v, f, bc, rot2d, scale2d
 
 
-@pytest.mark.parametrize("per_gaussian", [True, False])
-def test_stage3_forward_backward(per_gaussian):
-    print(f"case: bind stage3 per_gaussian={per_gaussian}", flush=True)
-    v, f, bc, rot2d, scale2d = _mesh(3, 1, per_gaussian=per_gaussian)
+@pytest.mark.parametrize("per_gaussian,sub", [(True, 3), (False, 3), (True, 6)])
+def test_stage3_forward_backward(per_gaussian, sub):
+    """sub 6 is the mesh-bound workload's size: 81 920 faces x 6 Gaussians."""
+    print(f"case: bind stage3 per_gaussian={per_gaussian} subdivisions={sub}", flush=True)
+    v, f, bc, rot2d, scale2d = _mesh(sub, 1, per_gaussian=per_gaussian)
     n = rot2d.shape[0]
     ov, orot, osc = [t.double().requires_grad_(True) for t in (v, rot2d, scale2d)]
     xyz_o, q_o, s_o, R_o = OB.stage3(ov, f, bc.double(), orot, osc, 1e-4)
