@@ -195,9 +195,10 @@ class BatchedRenderer:
             K_cap = max(1024, int(math.ceil(K * self.headroom)))
         self.K_cap = int(K_cap)
         self.lists = torch.empty(self.lib.dmgs_batched_lists_size(self.K_cap), dtype=torch.uint8, device=self.device)
-        if backward and self.bwd_ws is None:  # zero once; each full backward leaves it zeroed
-            self.bwd_ws = torch.zeros(self.lib.dmgs_batched_backward_size(self.B, self.P), dtype=torch.uint8,
+        if backward and self.bwd_ws is None:  # the forward blend clears it before each backward
+            self.bwd_ws = torch.empty(self.lib.dmgs_batched_backward_size(self.B, self.P), dtype=torch.uint8,
                                       device=self.device)
+            self._acc_clear = False
         return self.K_cap
 
     def _require_plan(self):
@@ -224,6 +225,8 @@ class BatchedRenderer:
             images = self.new_images()
         N.check(self.lib.dmgs_batched_forward(ctypes.byref(self._desc()), N.ptr(params), N.ptr(images), stages,
                                               N.stream_handle(self.device)), "dmgs_batched_forward")
+        if stages & STAGE_BLEND and self.bwd_ws is not None:
+            self._acc_clear = True  # the blend cleared the backward accumulator
         return images
 
     def backward(self, params, dL_dimages, grads=None, stages=STAGE_BLEND | STAGE_GEOMETRY):
@@ -237,6 +240,10 @@ class BatchedRenderer:
             raise ValueError("dL_dimages must be a contiguous buffer packed like the images")
         if grads is None:
             grads = torch.empty(CHANNELS, self.P, dtype=torch.float32, device=self.device)
+        if stages & STAGE_BLEND:
+            if not getattr(self, "_acc_clear", False):  # a second backward after one forward
+                self.bwd_ws.zero_()
+            self._acc_clear = False
         N.check(self.lib.dmgs_batched_backward(ctypes.byref(self._desc()), N.ptr(params), N.ptr(dL_dimages),
                                                N.ptr(grads), stages, N.stream_handle(self.device)),
                 "dmgs_batched_backward")

@@ -362,6 +362,10 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
   if (stages & DMGS_STAGE_BLEND) {
     dmgs::RenderArgs r = render_args(b, c);
     r.out_color = images;
+    if (b->bwd_ws) {  // the blend work-groups also clear the backward accumulator
+      r.clear = static_cast<float4*>(b->bwd_ws);
+      r.clear_n = c.N * DMGS_GACC_STRIDE / 4;
+    }
     dmgs::launch_render_fwd(r, st, b->variant);
     if ((rc = bcheck("batch render forward", st))) return rc;
   }

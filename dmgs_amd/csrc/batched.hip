@@ -170,11 +170,6 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
     const float4 g47 = *reinterpret_cast<const float4*>(gr + 4);
     const float g8 = gr[8];
     const ViewParams vp = view_params(const_ptr(a.views)[__builtin_amdgcn_readfirstlane(v)]);
-    // hand the row back zeroed: the next blend backward accumulates into zeros
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    reinterpret_cast<float4*>(gr)[0] = z;
-    reinterpret_cast<float4*>(gr)[1] = z;
-    reinterpret_cast<float4*>(gr)[2] = z;
     if (a.dens) {
       d_acc += sqrtf(g03.x * g03.x + g03.y * g03.y);
       d_cnt += 1.0f;

@@ -136,7 +136,8 @@ void launch_mv_compact(const MVArgs& a, const uint32_t* depth_keys, uint32_t* ke
 // (0 for i >= *n_dev when given).
 void launch_gather_rects(const uint2* rects, const uint32_t* order, uint2* rects_sorted, uint32_t* counts, size_t n,
                          hipStream_t s, const uint32_t* n_dev = nullptr);
-// Reads the accumulator rows of visible entries and leaves them zeroed.
+// Reads the accumulator rows of visible entries (the forward blend cleared
+// the accumulator; the blend backward filled it).
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s);
 
 // ---- blend ----
@@ -166,6 +167,10 @@ struct RenderArgs {
   // optional (multi-view, strip forward): per tile, the largest last
   // contributor of its pixels (the backward's list length)
   uint32_t* tile_work;
+  // optional (forward): the backward accumulator, zeroed by the forward's
+  // work-groups in equal shares after their blend work
+  float4* clear;
+  size_t clear_n;  // float4 count
 };
 // Work-group shape / LDS batch depth of the blend kernels (results are the
 // same for every variant; the autotuner picks the fastest).  The strip

@@ -233,6 +233,15 @@ __device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
 // order, and go on while another chunk surely fits in the BATCH slots.
 // Advances cursor; returns the number staged (workgroup-uniform); ends with
 // a barrier.  s_pos receives each staged instance's list position.
+// A forward work-group's share of clearing the backward accumulator: run
+// after its blend work, so the stores delay no load of the blend.
+__device__ __forceinline__ void clear_share(const RenderArgs& a) {
+  if (!a.clear) return;
+  const size_t per = (a.clear_n + gridDim.x - 1) / gridDim.x;
+  const size_t lo = (size_t)blockIdx.x * per, hi = min(a.clear_n, lo + per);
+  for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) a.clear[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 template <int NT, int BATCH, bool BACKWARD>
 __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_pos, uint32_t* s_wc,
                                                 const RenderArgs& a, const TileCtx& c, int& cursor, int limit) {
@@ -330,6 +339,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_kernel(RenderArgs a) {
     out[c.plane + pid] = C1 + T * c.bg1;
     out[2 * c.plane + pid] = C2 + T * c.bg2;
   }
+  clear_share(a);
 }
 
 // Forward over cell lists (multi-view).  Same per-pixel arithmetic and order
@@ -387,6 +397,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a
     out[c.plane + pid] = C1 + T * c.bg1;
     out[2 * c.plane + pid] = C2 + T * c.bg2;
   }
+  clear_share(a);
 }
 
 // Forward with one wave per 16x16 tile (the layout of the strip backward):
@@ -485,6 +496,7 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
     out[c.plane + pid] = C1[k] + Tk * c.bg1;
     out[2 * c.plane + pid] = C2[k] + Tk * c.bg2;
   }
+  clear_share(a);
 }
 
 // Backward.  CELLS: the lists are cell lists (multi-view); positions replace

@@ -265,10 +265,11 @@ int dmgs_batched_tile_view(const dmgs_view* views, int32_t B, int32_t* tile_view
  *           per-tile ranges; per-pixel blend state.  Kept until backward.
  *   lists   the (Gaussian, tile) instance lists for up to `capacity`
  *           instances and their sort scratch.  Kept until backward.
- *   bwd_ws  per (view, Gaussian) gradient accumulator (backward only).  It
- *           must hold zeros before its first use (clear it once after
- *           allocating); every backward that runs the GEOMETRY stage leaves
- *           it zeroed again, so no clearing pass runs per step. */
+ *   bwd_ws  per (view, Gaussian) gradient accumulator (backward only).  A
+ *           forward whose BLEND stage runs with bwd_ws set clears it (the
+ *           blend work-groups zero it after their blend work, so no clearing
+ *           pass runs per step); a backward accumulates into it, so every
+ *           backward needs a forward BLEND since the previous backward. */
 size_t dmgs_batched_state_size(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels);
 size_t dmgs_batched_lists_size(int64_t capacity);
 size_t dmgs_batched_backward_size(int32_t B, int64_t P);
