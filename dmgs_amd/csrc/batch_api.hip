@@ -87,8 +87,8 @@ struct StateLayout {
     status = o; o += align_up(4);
     tile_order = o; o += align_up(T * 4);
     tile_bucket = o; o += align_up(T);
-    bucket_hist = o; o += align_up(512 * 4);
     tile_work = o; o += align_up(T * 4);
+    bucket_hist = o; o += 512 * 4;  // directly before the ranges: the duplicate zeroes both in one run
     ranges = o; o += align_up(T * 8);
     final_T = o; o += align_up(X * 4);
     n_contrib = o; o += align_up(X * 4);
@@ -298,16 +298,17 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
   const size_t N = c.N;
   const dmgs::MVArgs a = mv_args(b, c, params);
   uint32_t* offs = at<uint32_t>(b->state, c.S.offsets);
+  uint32_t* status = at<uint32_t>(b->state, c.S.status);
   if (stages & DMGS_STAGE_GEOMETRY) {
-    dmgs::launch_mv_preprocess(a, st);
+    dmgs::MVArgs ap = a;
+    ap.status = status;  // zeroed by the preprocess, set by the compaction
+    dmgs::launch_mv_preprocess(ap, st);
     uint32_t* gs = at<uint32_t>(b->state, c.S.gscratch);
     // visible entries, in entry order, become the depth sort's input
     const size_t nw = dmgs::mv_wave_count_words((int)b->num_gaussians, b->num_views);
     dmgs::launch_inclusive_scan(a.wave_counts, a.wave_counts, nw, gs, st);
     const uint32_t* V_dev = a.wave_counts + (nw - 1);
     uint32_t* keys = at<uint32_t>(b->state, c.S.keys_a);
-    uint32_t* status = at<uint32_t>(b->state, c.S.status);
-    dmgs::launch_fill_u32(status, 1, 0u, st);
     dmgs::launch_mv_compact(a, a.depth_keys, keys, a.order, b->cell_mode == 1, status, st);
     dmgs::radix_sort_depth_keys(keys, a.order, at<uint32_t>(b->state, c.S.keys_b),
                                 at<uint32_t>(b->state, c.S.order_b), N, gs, st, V_dev);
@@ -331,6 +332,9 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     d.K_dev = c.count(b);
     d.chunk_first = at<uint32_t>(b->lists, c.L.chunk);
     d.view_major = b->cell_mode == 1;
+    // the duplicate also zeroes the tile-order histogram and the cell ranges (adjacent)
+    d.zero = at<uint32_t>(b->state, c.S.bucket_hist);
+    d.zero_n = 512 + 2 * (size_t)b->total_cells;
     // view-major lists: the duplicate also counts the first sort pass's digits
     const int w0 = d.view_major ? dmgs::radix_min_passes_first_width(K, c.local_bits) : -1;
     if (w0 > 0) {
@@ -347,7 +351,7 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
                                                           w0 > 0);
       if (in_b != c.lists_in_b()) return bfail(DMGS_EHIP, "internal: sorted list buffer mismatch");
       dmgs::launch_view_cell_ranges(in_b ? kb : d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells,
-                                    a.views, st, d.K_dev);
+                                    a.views, st, d.K_dev, true);
     } else {
       dmgs::radix_sort_pairs(d.tile_keys, d.values, kb, vb, K, bits_for((uint32_t)b->total_cells),
                              at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev);
@@ -356,7 +360,8 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     }
     // blend dispatch order: tiles with the longest lists first
     dmgs::launch_tile_order(render_args(b, c), at<uint32_t>(b->state, c.S.tile_order),
-                            at<uint8_t>(b->state, c.S.tile_bucket), at<uint32_t>(b->state, c.S.bucket_hist), st);
+                            at<uint8_t>(b->state, c.S.tile_bucket), at<uint32_t>(b->state, c.S.bucket_hist), st,
+                            true);
     if ((rc = bcheck("batch binning", st))) return rc;
   }
   if (stages & DMGS_STAGE_BLEND) {

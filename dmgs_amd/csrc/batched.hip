@@ -66,6 +66,7 @@ __device__ __forceinline__ void count_visible(const MVArgs& a, int v, bool visib
 
 __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.status && g == 0) *a.status = 0u;  // the compaction sets its flags after this kernel
   const bool live = g < a.P;  // every lane stays for the per-view ballots
   MVGaussian G;
   if (live) load_mv(a, g, G);

@@ -88,6 +88,9 @@ struct DupArgs {
   // a key), written per 4096-slot chunk into the sort scratch
   uint32_t* hist;
   int hist_bits;
+  // optional: words a later step needs zeroed (cleared in shares by the work-groups)
+  uint32_t* zero;
+  size_t zero_n;
 };
 size_t duplicate_scratch_words(size_t K);
 void launch_duplicate(const DupArgs& a, hipStream_t s);
@@ -99,7 +102,7 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 // (view, cell) run is contiguous; ranges are indexed by global cell id
 // (views[view].cell_base + cell).
 void launch_view_cell_ranges(const uint32_t* keys, size_t K, uint2* ranges, int num_cells, const ViewDesc* views,
-                             hipStream_t s, const uint32_t* K_dev);
+                             hipStream_t s, const uint32_t* K_dev, bool ranges_zeroed = false);
 
 // ---- multi-view per-Gaussian stages ----
 // Component-major scene parameters (channels x P).
@@ -123,6 +126,7 @@ struct MVArgs {
   float* gacc;               // (B*P, 16) blend-backward accumulator; zero between backward passes
   float* grads;              // (59, P)
   float* dens;               // optional (3, P): densification statistics (see dmgs_batch)
+  uint32_t* status;          // optional: zeroed by the preprocess (flags set by the compaction)
 };
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s);
 size_t mv_wave_count_words(int P, int B);
@@ -190,7 +194,8 @@ void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant = kRender
 // Multi-view: order the a.num_tiles tiles by decreasing list length (a
 // binned estimate of their cost) so the longest start first.  bucket: one
 // byte per tile, hist: 512 words of scratch.
-void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s);
+void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s,
+                       bool hist_zeroed = false);
 // The same with a.tile_work (written by the strip forward) as each tile's
 // cost: the backward's exact list length.
 void launch_tile_order_bwd(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s);

@@ -890,21 +890,22 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const uint8_t* __restri
 }
 
 static void tile_order_impl(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, bool use_work,
-                            hipStream_t s) {
+                            hipStream_t s, bool hist_zeroed) {
   if (a.num_tiles <= 0 || !a.views) return;
   const unsigned blocks = (unsigned)((a.num_tiles + 255) / 256);
-  launch_fill_u32(hist, 512, 0u, s);
+  if (!hist_zeroed) launch_fill_u32(hist, 512, 0u, s);
   hipLaunchKernelGGL(tile_bucket_kernel, dim3(blocks), dim3(256), 0, s, a, bucket, hist, use_work ? 1 : 0);
   hipLaunchKernelGGL(tile_order_kernel, dim3(blocks), dim3(256), 0, s, (const uint8_t*)bucket, hist, a.num_tiles,
                      order);
 }
 
-void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s) {
-  tile_order_impl(a, order, bucket, hist, false, s);
+void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s,
+                       bool hist_zeroed) {
+  tile_order_impl(a, order, bucket, hist, false, s, hist_zeroed);
 }
 
 void launch_tile_order_bwd(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s) {
-  if (a.tile_work) tile_order_impl(a, order, bucket, hist, true, s);
+  if (a.tile_work) tile_order_impl(a, order, bucket, hist, true, s, false);
 }
 
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {

@@ -474,6 +474,11 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
   const uint32_t K = device_count(a.K, a.K_dev);
   const uint32_t lo_slot = blockIdx.x * kDupChunk;
   const uint32_t hmask = (1u << a.hist_bits) - 1u;
+  if (a.zero) {  // this work-group's share of the words a later step needs zeroed
+    const size_t per = (a.zero_n + gridDim.x - 1) / gridDim.x;
+    const size_t zlo = (size_t)blockIdx.x * per, zhi = min(a.zero_n, zlo + per);
+    for (size_t i = zlo + threadIdx.x; i < zhi; i += 256) a.zero[i] = 0u;
+  }
   if (lo_slot >= K) {  // uniform per block: an empty chunk counts nothing
     if (a.hist && threadIdx.x <= hmask) a.hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = 0u;
     return;
@@ -687,8 +692,8 @@ void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int 
 }
 
 void launch_view_cell_ranges(const uint32_t* keys, size_t K, uint2* ranges, int num_cells, const ViewDesc* views,
-                             hipStream_t s, const uint32_t* K_dev) {
-  launch_fill_u32((uint32_t*)ranges, (size_t)num_cells * 2, 0u, s);
+                             hipStream_t s, const uint32_t* K_dev, bool ranges_zeroed) {
+  if (!ranges_zeroed) launch_fill_u32((uint32_t*)ranges, (size_t)num_cells * 2, 0u, s);
   const size_t threads = (K + 3) / 4;
   if (K) hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, keys, K,
                             ranges, (uint32_t)num_cells, K_dev, (uint32_t)DMGS_CELL_KEY_MASK, views);
