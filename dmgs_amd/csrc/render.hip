@@ -34,10 +34,14 @@
 
 namespace dmgs {
 
+// The blend evaluates alpha = o exp(power) as o exp2(p) with p = log2(e) power,
+// from the conic scaled once per staged splat: q = log2(e) (-a/2, -b, -c/2).
+constexpr float kLog2e = 1.44269504088896341f;
+
 template <int BATCH>
 struct SplatLDS {
   float2 xy[BATCH];
-  float4 con_o[BATCH];   // conic a, b, c, opacity
+  float4 qo[BATCH];      // scaled conic q, opacity
   float4 rgb_id[BATCH];  // r, g, b, id (bits)
   uint32_t mask[BATCH];  // pixel rectangles of the work-group the splat can reach (reach_mask)
 };
@@ -92,13 +96,23 @@ __device__ __forceinline__ uint32_t reach_mask(float cx, float cy, float a, floa
   return mask;
 }
 
-// power = -1/2 (a dx^2 + c dy^2) - b dx dy with a fixed rounding sequence
-// (explicit products, no compiler contraction), so every kernel variant, and
-// the backward's recomputation, gets bit-identical values.
-__device__ __forceinline__ float splat_power(const float4& co, float dx, float dy) {
-  const float q = fmaf(__fmul_rn(co.z, dy), dy, __fmul_rn(__fmul_rn(co.x, dx), dx));
-  return fmaf(-0.5f, q, -__fmul_rn(__fmul_rn(co.y, dx), dy));
+// p = log2(e) power = (q.x dx) dx + dy (q.y dx + q.z dy) for power = -1/2 (a dx^2
+// + c dy^2) - b dx dy.  The dx terms depend only on the splat and the pixel
+// column, so a lane owning several pixels of one column forms them once
+// (power_dx).  The rounding sequence is fixed (explicit products, no compiler
+// contraction), so every kernel variant, and the backward's recomputation,
+// gets bit-identical values.
+struct PowerDx {
+  float xx, xy;  // q.x dx dx, q.y dx
+};
+__device__ __forceinline__ PowerDx power_dx(const float4& q, float dx) {
+  return {__fmul_rn(__fmul_rn(q.x, dx), dx), __fmul_rn(q.y, dx)};
 }
+__device__ __forceinline__ float power_at(const PowerDx& d, float qz, float dy) {
+  return fmaf(fmaf(qz, dy, d.xy), dy, d.xx);
+}
+// exp2 on the hardware exponential (one instruction)
+__device__ __forceinline__ float splat_exp2(float p) { return __builtin_amdgcn_exp2f(p); }
 
 #define DMGS_DPP_ADD(v, ctrl) ((v) + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), (ctrl), 0xF, 0xF, true)))
 
@@ -130,12 +144,57 @@ __device__ __forceinline__ float row_sums9(const float (&v)[DMGS_GACC_USED], uns
   return row_lane == 8u ? s8 : sel;
 }
 
+// Backward state of one pixel, walking its list back to front.  The reference
+// keeps per channel ch an accumulator acc_ch (colour behind the current splat)
+// and needs only  sum_ch (col_ch - acc_ch) g_ch  (g = dL/dpixel), so one
+// recursion on S = sum_ch acc_ch g_ch replaces three:
+// S <- alpha' <col', g> + (1 - alpha') S, with alpha', col' the splat behind.
+struct BwdPixel {
+  float T;    // transmittance in front of the splat last processed
+  float S;    // sum_ch acc_ch g_ch
+  float pcg;  // alpha' <col', g>: the recursion's next term
+  float om;   // 1 - alpha': its factor
+};
+__device__ __forceinline__ BwdPixel bwd_pixel(float T_final) { return {T_final, 0.f, 0.f, 1.f}; }
+
+// One contributing splat (alpha, colour col) at a pixel with dL/dpixel =
+// (g0, g1, g2) and bg_term = -T_final <bg, g>: updates the state and returns
+// dL/dalpha; w = alpha T is dL/dcolour_ch / g_ch.
+__device__ __forceinline__ float bwd_step(BwdPixel& st, float alpha, const float4& col, float g0, float g1,
+                                          float g2, float bg_term, float& w) {
+  const float one_m = 1.f - alpha;
+  const float inv = __builtin_amdgcn_rcpf(one_m);  // T before this splat = T / (1 - alpha)
+  st.T *= inv;
+  w = alpha * st.T;
+  const float cg = fmaf(col.z, g2, fmaf(col.y, g1, col.x * g0));
+  st.S = fmaf(st.om, st.S, st.pcg);
+  const float dL_dalpha = fmaf(bg_term, inv, (cg - st.S) * st.T);
+  st.pcg = alpha * cg;
+  st.om = one_m;
+  return dL_dalpha;
+}
+
+// Gradients through G = exp(power), with t = dL/dG G = o G dL/dalpha (the
+// reference ignores the 0.99 clamp): dL/d(dx) = -t (a dx + b dy), dL/d(dy) =
+// -t (b dx + c dy), dL/d(a, b, c) = -t (dx^2, dx dy, dy^2) / 2, dL/do =
+// t / o.  From the scaled conic q = log2(e) (-a/2, -b, -c/2):
+// -(a X + b Y) = ln 2 (2 q.x X + q.y Y) and -(b X + c Y) = ln 2 (q.y X + 2 q.z Y).
+// The kernels accumulate, per splat, (2 q.x X + q.y Y, q.y X + 2 q.z Y,
+// sum t dx^2, sum t dx dy, sum t dy^2, sum G dL/dalpha, sum w g_ch) with
+// X = sum t dx, Y = sum t dy; the flush applies the constant factors.
+__device__ __forceinline__ float grad_scale(int k, float ddelx_dx, float ddely_dy) {
+  constexpr float kLn2 = 0.693147180559945309f;
+  return k == 0 ? kLn2 * ddelx_dx : (k == 1 ? kLn2 * ddely_dy : (k < 5 ? -0.5f : 1.0f));
+}
+__device__ __forceinline__ float dmean_x(const float4& q, float X, float Y) { return fmaf(q.x, X + X, q.y * Y); }
+__device__ __forceinline__ float dmean_y(const float4& q, float X, float Y) { return fmaf(q.z, Y + Y, q.y * X); }
+
 template <int BATCH>
 __device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const float4* __restrict__ splats,
                                             uint32_t id, uint32_t P, const RowRects& rr) {
   if (id >= P) {  // cannot happen for lists built by the binning step; a zero-opacity entry never contributes
     L.xy[slot] = make_float2(0.f, 0.f);
-    L.con_o[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    L.qo[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     L.rgb_id[slot] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
     L.mask[slot] = 0u;
     return;
@@ -144,8 +203,9 @@ __device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const 
   const float4 s0 = sp[0];
   const float4 s1 = sp[1];
   const float4 s2 = sp[2];
+  constexpr float kHalf = -0.5f * kLog2e;
   L.xy[slot] = make_float2(s0.x, s0.y);
-  L.con_o[slot] = make_float4(s0.z, s0.w, s1.x, s1.y);
+  L.qo[slot] = make_float4(kHalf * s0.z, -kLog2e * s0.w, kHalf * s1.x, s1.y);
   L.rgb_id[slot] = make_float4(s1.z, s1.w, s2.x, __uint_as_float(id));
   L.mask[slot] = reach_mask(s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, rr);
 }
@@ -242,7 +302,7 @@ __device__ __forceinline__ void clear_share(const RenderArgs& a) {
   for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) a.clear[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-template <int NT, int BATCH, bool BACKWARD>
+template <int NT, bool BACKWARD, int BATCH>
 __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_pos, uint32_t* s_wc,
                                                 const RenderArgs& a, const TileCtx& c, int& cursor, int limit) {
   constexpr int NW = NT / DMGS_WAVE;
@@ -310,11 +370,11 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_kernel(RenderArgs a) {
       if (!reaches(L.mask[j], my_rect)) continue;
       const uint32_t contributor = (uint32_t)(r * BATCH + j + 1);
       const float2 xy = L.xy[j];
-      const float4 co = L.con_o[j];
+      const float4 q = L.qo[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
-      const float power = splat_power(co, dx, dy);
-      if (power > 0.0f) continue;
-      const float alpha = fminf(0.99f, co.w * __expf(power));
+      const float p = power_at(power_dx(q, dx), q.z, dy);
+      if (p > 0.0f) continue;
+      const float alpha = fminf(0.99f, q.w * splat_exp2(p));
       if (alpha < 1.0f / 255.0f) continue;
       const float test_T = T * (1.0f - alpha);
       if (test_T < 0.0001f) {
@@ -364,15 +424,15 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a
   int cursor = 0;
   while (cursor < len) {
     if (__syncthreads_count(done) == NT) break;
-    const int n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, cursor, len);
+    const int n = stage_cell_batch<NT, false>(L, s_pos, s_wc, a, c, cursor, len);
     for (int j = 0; !done && j < n; ++j) {
       if (!reaches(L.mask[j], my_rect)) continue;
       const float2 xy = L.xy[j];
-      const float4 co = L.con_o[j];
+      const float4 q = L.qo[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
-      const float power = splat_power(co, dx, dy);
-      if (power > 0.0f) continue;
-      const float alpha = fminf(0.99f, co.w * __expf(power));
+      const float p = power_at(power_dx(q, dx), q.z, dy);
+      if (p > 0.0f) continue;
+      const float alpha = fminf(0.99f, q.w * splat_exp2(p));
       if (alpha < 1.0f / 255.0f) continue;
       const float test_T = T * (1.0f - alpha);
       if (test_T < 0.0001f) {
@@ -433,7 +493,7 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
     const int base = cursor;
     int n;
     if constexpr (CELLS) {
-      n = stage_cell_batch<NT, BATCH, false>(L, s_pos, s_wc, a, c, cursor, len);
+      n = stage_cell_batch<NT, false>(L, s_pos, s_wc, a, c, cursor, len);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = base + s;
@@ -449,30 +509,30 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
       if (j + 1 < n) mask_next = L.mask[j + 1];
       if (reach == 0u) continue;
       const float2 xy = L.xy[j];
-      const float4 co = L.con_o[j];
+      const float4 q = L.qo[j];
       uint32_t position;
       if constexpr (CELLS) {
         position = s_pos[j];
       } else {
         position = (uint32_t)(base + j);
       }
-      const float dx = xy.x - pxf;
+      const PowerDx pd = power_dx(q, xy.x - pxf);  // shared by the lane's four pixels (one column)
       const float4 col = L.rgb_id[j];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (!((reach >> k) & 1u) || __ballot(T[k] > 0.f) == 0) continue;  // wave-uniform
-        // the per-pixel tests become selects: no branch per pixel and splat
-        const float dy = xy.y - pyf[k];
-        const float power = splat_power(co, dx, dy);
-        const float alpha = fminf(0.99f, co.w * __expf(power));
+        // the per-pixel tests become selects: no branch per pixel and splat; a
+        // skipped splat adds colour x 0 (exact) and keeps T
+        const float p = power_at(pd, q.z, xy.y - pyf[k]);
+        const float alpha = fminf(0.99f, q.w * splat_exp2(p));
         const float test_T = T[k] * (1.0f - alpha);
-        const bool ok = T[k] > 0.f && power <= 0.0f && alpha >= 1.0f / 255.0f;
+        const bool ok = T[k] > 0.f && p <= 0.0f && alpha >= 1.0f / 255.0f;
         const bool stop = ok && test_T < 0.0001f;
         const bool take = ok && !stop;
-        const float w = alpha * T[k];
-        C0[k] = take ? fmaf(col.x, w, C0[k]) : C0[k];
-        C1[k] = take ? fmaf(col.y, w, C1[k]) : C1[k];
-        C2[k] = take ? fmaf(col.z, w, C2[k]) : C2[k];
+        const float w = take ? alpha * T[k] : 0.0f;
+        C0[k] = fmaf(col.x, w, C0[k]);
+        C1[k] = fmaf(col.y, w, C1[k]);
+        C2[k] = fmaf(col.z, w, C2[k]);
         T[k] = take ? test_T : (stop ? -T[k] : T[k]);
         last_contributor[k] = take ? position + 1 : last_contributor[k];
       }
@@ -537,10 +597,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
   for (int w = 0; w < NW; ++w) tile_max = max(tile_max, s_max[w]);
   const int todo_total = min((int)(c.range.y - c.range.x), (int)tile_max);
 
-  float T = T_final;
-  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f;
-  float last_alpha = 0.f, lc0 = 0.f, lc1 = 0.f, lc2 = 0.f;
-
+  BwdPixel st = bwd_pixel(T_final);
   const unsigned row_lane = lane & 15;  // lane within its DPP row of 16
   const int my_rect = SUB == 16 ? (int)(threadIdx.x >> 6) : 0;  // this wave's rectangle in the reach masks
   int cursor = todo_total;  // list positions below cursor are still to do
@@ -553,7 +610,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
-      n = stage_cell_batch<NT, BATCH, true>(L, s_pos, s_wc, a, c, cursor, 0);
+      n = stage_cell_batch<NT, true>(L, s_pos, s_wc, a, c, cursor, 0);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = top - 1 - s;
@@ -574,43 +631,29 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
         contributor = (uint32_t)(top - 1 - j);
       }
       const float2 xy = L.xy[j];
-      const float4 co = L.con_o[j];
+      const float4 q = L.qo[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
-      const float power = splat_power(co, dx, dy);
-      const float G = __expf(power);
-      const float alpha = fminf(0.99f, co.w * G);
-      const bool contrib = contributor < last_contributor && power <= 0.0f && alpha >= 1.0f / 255.0f;
+      const float p = power_at(power_dx(q, dx), q.z, dy);
+      const float G = splat_exp2(p);
+      const float ga = q.w * G;  // o G: alpha before the 0.99 clamp
+      const float alpha = fminf(0.99f, ga);
+      const bool contrib = contributor < last_contributor && p <= 0.0f && alpha >= 1.0f / 255.0f;
       if (__ballot(contrib) == 0) continue;  // wave-uniform
       float v[DMGS_GACC_USED];
       if (contrib) {
-        const float4 col = L.rgb_id[j];
-        const float inv_one_m = __builtin_amdgcn_rcpf(1.f - alpha);  // T before this splat = T / (1 - alpha)
-        T = T * inv_one_m;
-        const float dchannel_dcolor = alpha * T;
-        acc0 = last_alpha * lc0 + (1.f - last_alpha) * acc0;
-        acc1 = last_alpha * lc1 + (1.f - last_alpha) * acc1;
-        acc2 = last_alpha * lc2 + (1.f - last_alpha) * acc2;
-        lc0 = col.x;
-        lc1 = col.y;
-        lc2 = col.z;
-        float dL_dalpha = (col.x - acc0) * g0 + (col.y - acc1) * g1 + (col.z - acc2) * g2;
-        v[6] = dchannel_dcolor * g0;
-        v[7] = dchannel_dcolor * g1;
-        v[8] = dchannel_dcolor * g2;
-        dL_dalpha *= T;
-        last_alpha = alpha;
-        dL_dalpha += bg_term * inv_one_m;
-        const float dL_dG = co.w * dL_dalpha;
-        const float gdx = G * dx, gdy = G * dy;
-        const float dG_ddelx = -gdx * co.x - gdy * co.y;
-        const float dG_ddely = -gdy * co.z - gdx * co.y;
-        // constant factors (W/2, H/2, -1/2) are applied once per splat at the flush
-        v[0] = dL_dG * dG_ddelx;
-        v[1] = dL_dG * dG_ddely;
-        v[2] = gdx * dx * dL_dG;
-        v[3] = gdx * dy * dL_dG;
-        v[4] = gdy * dy * dL_dG;
+        float w;
+        const float dL_dalpha = bwd_step(st, alpha, L.rgb_id[j], g0, g1, g2, bg_term, w);
+        const float t = ga * dL_dalpha;
+        const float tdx = t * dx, tdy = t * dy;
+        v[0] = dmean_x(q, tdx, tdy);
+        v[1] = dmean_y(q, tdx, tdy);
+        v[2] = tdx * dx;
+        v[3] = tdx * dy;
+        v[4] = tdy * dy;
         v[5] = G * dL_dalpha;
+        v[6] = w * g0;
+        v[7] = w * g1;
+        v[8] = w * g2;
       } else {
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) v[k] = 0.f;
@@ -624,8 +667,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
     // flush the batch: consecutive lanes cover the 9 values of one splat
     for (int i = threadIdx.x; i < n * DMGS_GACC_USED; i += NT) {
       const int slot = i / DMGS_GACC_USED, k = i - slot * DMGS_GACC_USED;
-      const float scale = k == 0 ? ddelx_dx : (k == 1 ? ddely_dy : (k < 5 ? -0.5f : 1.0f));
-      const float val = s_acc[slot][k] * scale;
+      const float val = s_acc[slot][k] * grad_scale(k, ddelx_dx, ddely_dy);
       if (val != 0.f) {
         const uint32_t gid = __float_as_uint(L.rgb_id[slot].w);
         unsafeAtomicAdd(a.gacc + (size_t)gid * DMGS_GACC_STRIDE + k, val);
@@ -659,44 +701,40 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) pyf[k] = (float)(c.py + 4 * k);
 
-  float T[4], T_final[4], g0[4], g1[4], g2[4], bg_term[4];
-  // pc = last_alpha * last colour: the recursion's next term, formed once
-  float acc0[4], acc1[4], acc2[4], last_alpha[4], pc0[4], pc1[4], pc2[4];
+  float g0[4], g1[4], g2[4], bg_term[4];
+  BwdPixel st[4];
   uint32_t last_contributor[4];
   uint32_t m = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int py = c.py + 4 * k;
     const bool inside = c.px < c.W && py < c.H;
-    T_final[k] = 0.f;
+    float T_final = 0.f;
     g0[k] = g1[k] = g2[k] = 0.f;
     last_contributor[k] = 0;
     if (inside) {
       const size_t pix = c.pix + (size_t)(4 * k) * c.W;
       const size_t pid = pix - c.img / 3;
       const float* dp = a.dL_dpix + c.img;
-      T_final[k] = a.final_T[pix];
+      T_final = a.final_T[pix];
       last_contributor[k] = a.n_contrib[pix];
       g0[k] = dp[pid];
       g1[k] = dp[c.plane + pid];
       g2[k] = dp[2 * c.plane + pid];
     }
-    bg_term[k] = -T_final[k] * (c.bg0 * g0[k] + c.bg1 * g1[k] + c.bg2 * g2[k]);
-    T[k] = T_final[k];
-    acc0[k] = acc1[k] = acc2[k] = 0.f;
-    last_alpha[k] = pc0[k] = pc1[k] = pc2[k] = 0.f;
+    bg_term[k] = -T_final * (c.bg0 * g0[k] + c.bg1 * g1[k] + c.bg2 * g2[k]);
+    st[k] = bwd_pixel(T_final);
     m = max(m, last_contributor[k]);
   }
   for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
   const int todo_total = min((int)(c.range.y - c.range.x), (int)m);
 
   const unsigned row_lane = lane & 15;
-  // one batch's accumulated values -> one global atomic per (splat, value)
+  // one batch's accumulated sums -> one global atomic per (splat, value)
   auto flush = [&](int buf, int cnt) {
     for (int i = threadIdx.x; i < cnt * DMGS_GACC_USED; i += NT) {
       const int slot = i / DMGS_GACC_USED, q = i - slot * DMGS_GACC_USED;
-      const float scale = q == 0 ? ddelx_dx : (q == 1 ? ddely_dy : (q < 5 ? -0.5f : 1.0f));
-      const float val = s_acc[buf][slot][q] * scale;
+      const float val = s_acc[buf][slot][q] * grad_scale(q, ddelx_dx, ddely_dy);
       if (val != 0.f) unsafeAtomicAdd(a.gacc + (size_t)s_gid[buf][slot] * DMGS_GACC_STRIDE + q, val);
     }
   };
@@ -711,7 +749,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[buf][s][k] = 0.f;
     }
     if constexpr (CELLS) {
-      n = stage_cell_batch<NT, BATCH, true>(L, s_pos, s_wc, a, c, cursor, 0);
+      n = stage_cell_batch<NT, true>(L, s_pos, s_wc, a, c, cursor, 0);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = top - 1 - s;
@@ -739,52 +777,39 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
         contributor = (uint32_t)(top - 1 - j);
       }
       const float2 xy = L.xy[j];
-      const float4 co = L.con_o[j];
+      const float4 q = L.qo[j];
       const float4 col = L.rgb_id[j];
-      float v[DMGS_GACC_USED];
-#pragma unroll
-      for (int q = 0; q < DMGS_GACC_USED; ++q) v[q] = 0.f;
+      const float dx = xy.x - pxf;
+      const PowerDx pd = power_dx(q, dx);  // shared by the lane's four pixels (one column)
+      // per-lane sums over its pixels: sum t, sum t dy, sum t dy^2 (dx is the
+      // same for the four pixels, so its powers multiply in once per splat)
+      float s_t = 0.f, s_tdy = 0.f, s_tdy2 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
       bool any = false;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         if (!((reach >> k) & 1u)) continue;  // wave-uniform
-        const float dx = xy.x - pxf, dy = xy.y - pyf[k];
-        const float power = splat_power(co, dx, dy);
-        const float G = __expf(power);
-        const float alpha = fminf(0.99f, co.w * G);
-        if (contributor < last_contributor[k] && power <= 0.0f && alpha >= 1.0f / 255.0f) {
+        const float dy = xy.y - pyf[k];
+        const float p = power_at(pd, q.z, dy);
+        const float G = splat_exp2(p);
+        const float ga = q.w * G;  // o G: alpha before the 0.99 clamp
+        const float alpha = fminf(0.99f, ga);
+        if (contributor < last_contributor[k] && p <= 0.0f && alpha >= 1.0f / 255.0f) {
           any = true;
-          const float inv_one_m = __builtin_amdgcn_rcpf(1.f - alpha);
-          T[k] = T[k] * inv_one_m;
-          const float dchannel_dcolor = alpha * T[k];
-          const float om = 1.f - last_alpha[k];
-          acc0[k] = fmaf(om, acc0[k], pc0[k]);
-          acc1[k] = fmaf(om, acc1[k], pc1[k]);
-          acc2[k] = fmaf(om, acc2[k], pc2[k]);
-          pc0[k] = alpha * col.x;
-          pc1[k] = alpha * col.y;
-          pc2[k] = alpha * col.z;
-          float dL_dalpha = (col.x - acc0[k]) * g0[k] + (col.y - acc1[k]) * g1[k] + (col.z - acc2[k]) * g2[k];
-          v[6] += dchannel_dcolor * g0[k];
-          v[7] += dchannel_dcolor * g1[k];
-          v[8] += dchannel_dcolor * g2[k];
-          dL_dalpha *= T[k];
-          last_alpha[k] = alpha;
-          dL_dalpha += bg_term[k] * inv_one_m;
-          // with t = dL/dG * G: dG/d(dx) = -G (a dx + b dy), dG/d(dy) = -G (b dx + c dy),
-          // dG/d(a, b, c) = G (dx^2, dx dy, dy^2) times the constant factors
-          // (W/2, H/2, -1/2) applied once per splat at the flush
-          const float t = co.w * dL_dalpha * G;
-          const float tdx = t * dx, tdy = t * dy;
-          v[0] = fmaf(-tdy, co.y, fmaf(-tdx, co.x, v[0]));
-          v[1] = fmaf(-tdx, co.y, fmaf(-tdy, co.z, v[1]));
-          v[2] = fmaf(tdx, dx, v[2]);
-          v[3] = fmaf(tdx, dy, v[3]);
-          v[4] = fmaf(tdy, dy, v[4]);
-          v[5] += G * dL_dalpha;
+          float w;
+          const float t = ga * bwd_step(st[k], alpha, col, g0[k], g1[k], g2[k], bg_term[k], w);
+          c0 = fmaf(w, g0[k], c0);
+          c1 = fmaf(w, g1[k], c1);
+          c2 = fmaf(w, g2[k], c2);
+          s_t += t;
+          const float tdy = t * dy;
+          s_tdy += tdy;
+          s_tdy2 = fmaf(tdy, dy, s_tdy2);
         }
       }
       if (__ballot(any) == 0) continue;
+      const float s_tdx = s_t * dx;
+      const float v[DMGS_GACC_USED] = {dmean_x(q, s_tdx, s_tdy), dmean_y(q, s_tdx, s_tdy), s_tdx * dx, s_tdy * dx,
+                                       s_tdy2, s_t * __builtin_amdgcn_rcpf(q.w), c0, c1, c2};
       const float mine = row_sums9(v, row_lane);
       if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[buf][j][row_lane], mine);
     }
