@@ -65,7 +65,9 @@ struct RowRects {  // n rectangles [x0, x1] x [y0 + k h, y0 + k h + h - 1]
 __device__ __forceinline__ bool ellipse_meets_rect(float cx, float cy, float a, float b, float c, float tau,
                                                    float x0, float x1, float y0, float y1) {
   if (cx >= x0 && cx <= x1 && cy >= y0 && cy <= y1) return true;
-  const float ia = 1.0f / a, ic = 1.0f / c;
+  // (an approximate reciprocal only moves the edge point off the exact
+  // minimiser, which raises the value found by far less than the margins)
+  const float ia = __builtin_amdgcn_rcpf(a), ic = __builtin_amdgcn_rcpf(c);
   float best = INFINITY;
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
@@ -122,26 +124,33 @@ __device__ __forceinline__ bool reaches(uint32_t mask_lds, int k) {
 }
 
 // Row sums of nine per-lane values over each 16-lane DPP row; row lane k < 9
-// returns the sum of value k.  Three butterfly steps (quad swaps, rotate by 4)
-// on all nine values leave every lane the sum over its half of the row; each
-// lane then keeps value (lane & 7), so one rotate-by-8 step completes values
-// 0..7 (the partner lane holds the other half of the same value), and value
-// 8 takes one step of its own.  37 VALU instead of 54 for nine full row sums.
+// returns the sum of value k.  Two reduce-scatter steps over lane bits 0 and
+// 1 (each lane keeps the half of the values its partner gives up: 3 VALU per
+// pair) leave lane m (m = lane & 3) the quad sums of values m and 4 + m;
+// value 8 takes plain butterfly steps; rotating the row by 4 and by 8 then
+// adds the four quads.  28 VALU instead of 54 for nine full row sums.
 __device__ __forceinline__ float row_sums9(const float (&v)[DMGS_GACC_USED], unsigned row_lane) {
-  float h[DMGS_GACC_USED];
+  const bool b0 = (row_lane & 1u) != 0u, b1 = (row_lane & 2u) != 0u;
+  float p[4];
 #pragma unroll
-  for (int k = 0; k < DMGS_GACC_USED; ++k) {
-    float x = DMGS_DPP_ADD(v[k], 0xB1);  // quad_perm 1,0,3,2
-    x = DMGS_DPP_ADD(x, 0x4E);           // quad_perm 2,3,0,1
-    h[k] = DMGS_DPP_ADD(x, 0x124);       // row_ror 4
+  for (int i = 0; i < 4; ++i) {  // lanes l, l ^ 1: keep value 2i + b0
+    const float keep = b0 ? v[2 * i + 1] : v[2 * i];
+    const float give = b0 ? v[2 * i] : v[2 * i + 1];
+    p[i] = keep + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(give), 0xB1, 0xF, 0xF, true));
   }
-  const unsigned sel_k = row_lane & 7u;
-  float sel = h[0];
+  float e = DMGS_DPP_ADD(v[8], 0xB1);
+  float r[2];
 #pragma unroll
-  for (int k = 1; k < 8; ++k) sel = (sel_k == (unsigned)k) ? h[k] : sel;
-  sel = DMGS_DPP_ADD(sel, 0x128);        // row_ror 8
-  const float s8 = DMGS_DPP_ADD(h[8], 0x128);
-  return row_lane == 8u ? s8 : sel;
+  for (int i = 0; i < 2; ++i) {  // lanes l, l ^ 2: keep value 4i + 2 b1 + b0
+    const float keep = b1 ? p[2 * i + 1] : p[2 * i];
+    const float give = b1 ? p[2 * i] : p[2 * i + 1];
+    r[i] = keep + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(give), 0x4E, 0xF, 0xF, true));
+  }
+  e = DMGS_DPP_ADD(e, 0x4E);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) r[i] = DMGS_DPP_ADD(DMGS_DPP_ADD(r[i], 0x124), 0x128);  // row_ror 4, row_ror 8
+  e = DMGS_DPP_ADD(DMGS_DPP_ADD(e, 0x124), 0x128);
+  return row_lane < 4u ? r[0] : (row_lane < 8u ? r[1] : e);
 }
 
 // Backward state of one pixel, walking its list back to front.  The reference
@@ -162,16 +171,15 @@ __device__ __forceinline__ BwdPixel bwd_pixel(float T_final) { return {T_final, 
 // dL/dalpha; w = alpha T is dL/dcolour_ch / g_ch.
 __device__ __forceinline__ float bwd_step(BwdPixel& st, float alpha, const float4& col, float g0, float g1,
                                           float g2, float bg_term, float& w) {
-  const float one_m = 1.f - alpha;
-  const float inv = __builtin_amdgcn_rcpf(one_m);  // T before this splat = T / (1 - alpha)
+  // each state value is read before it is replaced, so it can be updated in place
+  st.S = fmaf(st.om, st.S, st.pcg);
+  st.om = 1.f - alpha;
+  const float inv = __builtin_amdgcn_rcpf(st.om);  // T before this splat = T / (1 - alpha)
   st.T *= inv;
   w = alpha * st.T;
   const float cg = fmaf(col.z, g2, fmaf(col.y, g1, col.x * g0));
-  st.S = fmaf(st.om, st.S, st.pcg);
-  const float dL_dalpha = fmaf(bg_term, inv, (cg - st.S) * st.T);
   st.pcg = alpha * cg;
-  st.om = one_m;
-  return dL_dalpha;
+  return fmaf(bg_term, inv, (cg - st.S) * st.T);
 }
 
 // Gradients through G = exp(power), with t = dL/dG G = o G dL/dalpha (the
@@ -286,13 +294,6 @@ __device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
   return c;
 }
 
-// Stage the next instances of this tile from its cell list (multi-view):
-// visit list positions from `cursor` — forward: cursor, cursor+1, ... below
-// limit; backward: cursor-1, cursor-2, ... down to 0 — min(NT, BATCH) at a
-// time, keep those whose key holds this tile's bit, stage them in visit
-// order, and go on while another chunk surely fits in the BATCH slots.
-// Advances cursor; returns the number staged (workgroup-uniform); ends with
-// a barrier.  s_pos receives each staged instance's list position.
 // A forward work-group's share of clearing the backward accumulator: run
 // after its blend work, so the stores delay no load of the blend.
 __device__ __forceinline__ void clear_share(const RenderArgs& a) {
@@ -302,6 +303,15 @@ __device__ __forceinline__ void clear_share(const RenderArgs& a) {
   for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) a.clear[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// Stage the next instances of this tile from its cell list (multi-view):
+// visit list positions from `cursor` — forward: cursor, cursor+1, ... below
+// limit; backward: cursor-1, cursor-2, ... down to 0 — min(NT, BATCH) at a
+// time and keep those whose key holds this tile's bit, in visit order, while
+// another chunk surely fits in the BATCH slots; then stage the kept records
+// in one pass, one lane per record (the footprint test then runs once per
+// NT records instead of once per chunk visited).  Advances cursor; returns
+// the number staged (workgroup-uniform); ends with a barrier.  s_pos
+// receives each staged instance's list position.
 template <int NT, bool BACKWARD, int BATCH>
 __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_pos, uint32_t* s_wc,
                                                 const RenderArgs& a, const TileCtx& c, int& cursor, int limit) {
@@ -322,24 +332,32 @@ __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_
     const uint32_t id = a.point_list[at];
     const bool member = valid && ((key >> DMGS_CELL_KEY_BITS) & c.bit);
     const uint64_t bal = __ballot(member);
-    if (lane == 0) s_wc[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    int before = run, total = 0;
+    int before = run, total;
+    if constexpr (NW == 1) {
+      total = (int)__popcll(bal);
+    } else {
+      if (lane == 0) s_wc[wave] = (uint32_t)__popcll(bal);
+      __syncthreads();
+      total = 0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) {
-      const int cnt = (int)s_wc[w];
-      before += (w < (int)wave) ? cnt : 0;
-      total += cnt;
+      for (int w = 0; w < NW; ++w) {
+        const int cnt = (int)s_wc[w];
+        before += (w < (int)wave) ? cnt : 0;
+        total += cnt;
+      }
     }
     if (member) {
       const int slot = before + (int)__popcll(bal & lt);
-      stage_splat(L, slot, a.splats, id, a.P, c.rows);
+      L.mask[slot] = id;  // the id waits here for the staging pass
       s_pos[slot] = (uint32_t)q;
     }
     run += total;
     cursor = BACKWARD ? cursor - CH : cursor + CH;
-    __syncthreads();
+    if constexpr (NW > 1) __syncthreads();  // s_wc is rewritten by the next chunk
   } while (run + CH <= BATCH && (BACKWARD ? cursor > 0 : cursor < limit));
+  __syncthreads();
+  for (int s = (int)threadIdx.x; s < run; s += NT) stage_splat(L, s, a.splats, L.mask[s], a.P, c.rows);
+  __syncthreads();
   return run;
 }
 
