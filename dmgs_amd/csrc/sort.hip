@@ -668,16 +668,21 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, 
   for (int j = 0; j < 4; ++j) {
     const size_t i = i0 + j;
     if (i >= K) break;
-    const uint32_t t = range_slot(k[j], views);
-    const uint32_t pt = range_slot(prev, views);
-    if (t < num_tiles) {  // always true for keys from the duplicate step
-      if (i == 0) {
-        ranges[t].x = 0;
-      } else if (k[j] != prev && pt < num_tiles) {
-        ranges[pt].y = (uint32_t)i;
-        ranges[t].x = (uint32_t)i;
+    // boundaries are rare: only they look up range slots (a table read with views)
+    if (i == 0 || i == K - 1 || k[j] != prev) {
+      const uint32_t t = range_slot(k[j], views);
+      if (t < num_tiles) {  // always true for keys from the duplicate step
+        if (i == 0) {
+          ranges[t].x = 0;
+        } else if (k[j] != prev) {
+          const uint32_t pt = range_slot(prev, views);
+          if (pt < num_tiles) {
+            ranges[pt].y = (uint32_t)i;
+            ranges[t].x = (uint32_t)i;
+          }
+        }
+        if (i == K - 1) ranges[t].y = (uint32_t)K;
       }
-      if (i == K - 1) ranges[t].y = (uint32_t)K;
     }
     prev = k[j];
   }
