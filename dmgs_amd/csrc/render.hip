@@ -38,13 +38,22 @@ namespace dmgs {
 // from the conic scaled once per staged splat: q = log2(e) (-a/2, -b, -c/2).
 constexpr float kLog2e = 1.44269504088896341f;
 
+// Three 16-byte fields per staged splat, all indexed alike: one address
+// register serves every field of a splat (fixed offsets between them).
 template <int BATCH>
 struct SplatLDS {
-  float2 xy[BATCH];
+  float4 xym[BATCH];     // x, y, reach mask (bits, reach_mask), list position (bits)
   float4 qo[BATCH];      // scaled conic q, opacity
   float4 rgb_id[BATCH];  // r, g, b, id (bits)
-  uint32_t mask[BATCH];  // pixel rectangles of the work-group the splat can reach (reach_mask)
 };
+template <int BATCH>
+__device__ __forceinline__ uint32_t splat_mask(const SplatLDS<BATCH>& L, int j) {
+  return __float_as_uint(L.xym[j].z);
+}
+template <int BATCH>
+__device__ __forceinline__ uint32_t splat_pos(const SplatLDS<BATCH>& L, int j) {
+  return __float_as_uint(L.xym[j].w);
+}
 
 // Which pixel rectangles of the work-group a splat can reach.  A splat passes
 // the alpha test only inside the ellipse  d^T Q d <= tau,  tau = 2 ln(255 o)
@@ -199,12 +208,11 @@ __device__ __forceinline__ float dmean_y(const float4& q, float X, float Y) { re
 
 template <int BATCH>
 __device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const float4* __restrict__ splats,
-                                            uint32_t id, uint32_t P, const RowRects& rr) {
+                                            uint32_t id, uint32_t P, const RowRects& rr, uint32_t pos) {
   if (id >= P) {  // cannot happen for lists built by the binning step; a zero-opacity entry never contributes
-    L.xy[slot] = make_float2(0.f, 0.f);
+    L.xym[slot] = make_float4(0.f, 0.f, __uint_as_float(0u), __uint_as_float(pos));
     L.qo[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
     L.rgb_id[slot] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
-    L.mask[slot] = 0u;
     return;
   }
   const float4* sp = splats + (size_t)DMGS_SPLAT_F4 * id;
@@ -212,10 +220,10 @@ __device__ __forceinline__ void stage_splat(SplatLDS<BATCH>& L, int slot, const 
   const float4 s1 = sp[1];
   const float4 s2 = sp[2];
   constexpr float kHalf = -0.5f * kLog2e;
-  L.xy[slot] = make_float2(s0.x, s0.y);
+  const uint32_t mask = reach_mask(s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, rr);
+  L.xym[slot] = make_float4(s0.x, s0.y, __uint_as_float(mask), __uint_as_float(pos));
   L.qo[slot] = make_float4(kHalf * s0.z, -kLog2e * s0.w, kHalf * s1.x, s1.y);
   L.rgb_id[slot] = make_float4(s1.z, s1.w, s2.x, __uint_as_float(id));
-  L.mask[slot] = reach_mask(s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, rr);
 }
 
 // Everything a workgroup needs to know about its pixels.
@@ -310,10 +318,10 @@ __device__ __forceinline__ void clear_share(const RenderArgs& a) {
 // another chunk surely fits in the BATCH slots; then stage the kept records
 // in one pass, one lane per record (the footprint test then runs once per
 // NT records instead of once per chunk visited).  Advances cursor; returns
-// the number staged (workgroup-uniform); ends with a barrier.  s_pos
-// receives each staged instance's list position.
+// the number staged (workgroup-uniform); ends with a barrier.  Each staged
+// record carries its list position.
 template <int NT, bool BACKWARD, int BATCH>
-__device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_pos, uint32_t* s_wc,
+__device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_wc,
                                                 const RenderArgs& a, const TileCtx& c, int& cursor, int limit) {
   constexpr int NW = NT / DMGS_WAVE;
   constexpr int CH = NT < BATCH ? NT : BATCH;  // positions per chunk
@@ -348,15 +356,18 @@ __device__ __forceinline__ int stage_cell_batch(SplatLDS<BATCH>& L, uint32_t* s_
     }
     if (member) {
       const int slot = before + (int)__popcll(bal & lt);
-      L.mask[slot] = id;  // the id waits here for the staging pass
-      s_pos[slot] = (uint32_t)q;
+      // id and position wait in the record for the staging pass
+      L.xym[slot] = make_float4(0.f, 0.f, __uint_as_float(id), __uint_as_float((uint32_t)q));
     }
     run += total;
     cursor = BACKWARD ? cursor - CH : cursor + CH;
     if constexpr (NW > 1) __syncthreads();  // s_wc is rewritten by the next chunk
   } while (run + CH <= BATCH && (BACKWARD ? cursor > 0 : cursor < limit));
   __syncthreads();
-  for (int s = (int)threadIdx.x; s < run; s += NT) stage_splat(L, s, a.splats, L.mask[s], a.P, c.rows);
+  for (int s = (int)threadIdx.x; s < run; s += NT) {
+    const float4 t = L.xym[s];
+    stage_splat(L, s, a.splats, __float_as_uint(t.z), a.P, c.rows, __float_as_uint(t.w));
+  }
   __syncthreads();
   return run;
 }
@@ -380,14 +391,14 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_kernel(RenderArgs a) {
     if (__syncthreads_count(done) == NT) break;
     for (int s = threadIdx.x; s < BATCH; s += NT) {
       const int pos = r * BATCH + s;
-      if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
+      if (pos < todo_total) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows, (uint32_t)pos);
     }
     __syncthreads();
     const int n = min(BATCH, todo_total - r * BATCH);
     for (int j = 0; !done && j < n; ++j) {
-      if (!reaches(L.mask[j], my_rect)) continue;
+      if (!reaches(splat_mask(L, j), my_rect)) continue;
       const uint32_t contributor = (uint32_t)(r * BATCH + j + 1);
-      const float2 xy = L.xy[j];
+      const float2 xy = make_float2(L.xym[j].x, L.xym[j].y);
       const float4 q = L.qo[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
       const float p = power_at(power_dx(q, dx), q.z, dy);
@@ -427,7 +438,6 @@ template <int SUB, int BATCH>
 __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a) {
   constexpr int NT = SUB * SUB;
   __shared__ SplatLDS<BATCH> L;
-  __shared__ uint32_t s_pos[BATCH];
   __shared__ uint32_t s_wc[NT / DMGS_WAVE];
   const TileCtx c = tile_ctx<SUB>(a);
   const float pxf = (float)c.px, pyf = (float)c.py;
@@ -442,10 +452,10 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a
   int cursor = 0;
   while (cursor < len) {
     if (__syncthreads_count(done) == NT) break;
-    const int n = stage_cell_batch<NT, false>(L, s_pos, s_wc, a, c, cursor, len);
+    const int n = stage_cell_batch<NT, false>(L, s_wc, a, c, cursor, len);
     for (int j = 0; !done && j < n; ++j) {
-      if (!reaches(L.mask[j], my_rect)) continue;
-      const float2 xy = L.xy[j];
+      if (!reaches(splat_mask(L, j), my_rect)) continue;
+      const float2 xy = make_float2(L.xym[j].x, L.xym[j].y);
       const float4 q = L.qo[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
       const float p = power_at(power_dx(q, dx), q.z, dy);
@@ -463,7 +473,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_fwd_cells_kernel(RenderArgs a
       C1 += col.y * w;
       C2 += col.z * w;
       T = test_T;
-      last_contributor = s_pos[j] + 1;
+      last_contributor = splat_pos(L, j) + 1;
     }
   }
   if (c.inside) {
@@ -487,7 +497,6 @@ template <int BATCH, bool CELLS>
 __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
   constexpr int NT = 64;
   __shared__ SplatLDS<BATCH> L;
-  __shared__ uint32_t s_pos[CELLS ? BATCH : 1];
   __shared__ uint32_t s_wc[1];
   const TileCtx c = tile_ctx<16>(a);
   const float pxf = (float)c.px;
@@ -511,26 +520,26 @@ __global__ __launch_bounds__(64) void render_fwd_strips_kernel(RenderArgs a) {
     const int base = cursor;
     int n;
     if constexpr (CELLS) {
-      n = stage_cell_batch<NT, false>(L, s_pos, s_wc, a, c, cursor, len);
+      n = stage_cell_batch<NT, false>(L, s_wc, a, c, cursor, len);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = base + s;
-        if (pos < len) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
+        if (pos < len) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows, (uint32_t)pos);
       }
       __syncthreads();
       n = min(BATCH, len - base);
       cursor += n;
     }
-    uint32_t mask_next = n > 0 ? L.mask[0] : 0u;  // read one splat ahead
+    uint32_t mask_next = n > 0 ? splat_mask(L, 0) : 0u;  // read one splat ahead
     for (int j = 0; j < n; ++j) {
       const uint32_t reach = __builtin_amdgcn_readfirstlane(mask_next);
-      if (j + 1 < n) mask_next = L.mask[j + 1];
+      if (j + 1 < n) mask_next = splat_mask(L, j + 1);
       if (reach == 0u) continue;
-      const float2 xy = L.xy[j];
+      const float2 xy = make_float2(L.xym[j].x, L.xym[j].y);
       const float4 q = L.qo[j];
       uint32_t position;
       if constexpr (CELLS) {
-        position = s_pos[j];
+        position = splat_pos(L, j);
       } else {
         position = (uint32_t)(base + j);
       }
@@ -586,7 +595,6 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
   __shared__ SplatLDS<BATCH> L;
   __shared__ uint32_t s_max[NW];
   __shared__ float s_acc[BATCH][DMGS_GACC_USED];
-  __shared__ uint32_t s_pos[CELLS ? BATCH : 1];
   __shared__ uint32_t s_wc[NW];
   const TileCtx c = tile_ctx<SUB>(a);
   const float pxf = (float)c.px, pyf = (float)c.py;
@@ -628,11 +636,11 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
-      n = stage_cell_batch<NT, true>(L, s_pos, s_wc, a, c, cursor, 0);
+      n = stage_cell_batch<NT, true>(L, s_wc, a, c, cursor, 0);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = top - 1 - s;
-        if (pos >= 0) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
+        if (pos >= 0) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows, (uint32_t)pos);
 #pragma unroll
         for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[s][k] = 0.f;
       }
@@ -641,14 +649,14 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
       cursor -= n;
     }
     for (int j = 0; j < n; ++j) {
-      if (!reaches(L.mask[j], my_rect)) continue;  // no pixel of this wave can pass the alpha test
+      if (!reaches(splat_mask(L, j), my_rect)) continue;  // no pixel of this wave can pass the alpha test
       uint32_t contributor;
       if constexpr (CELLS) {
-        contributor = s_pos[j];
+        contributor = splat_pos(L, j);
       } else {
         contributor = (uint32_t)(top - 1 - j);
       }
-      const float2 xy = L.xy[j];
+      const float2 xy = make_float2(L.xym[j].x, L.xym[j].y);
       const float4 q = L.qo[j];
       const float dx = xy.x - pxf, dy = xy.y - pyf;
       const float p = power_at(power_dx(q, dx), q.z, dy);
@@ -709,7 +717,6 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   // counter is in order) and they complete while that batch is computed.
   __shared__ float s_acc[2][BATCH][DMGS_GACC_USED];
   __shared__ uint32_t s_gid[2][BATCH];
-  __shared__ uint32_t s_pos[CELLS ? BATCH : 1];
   __shared__ uint32_t s_wc[1];
   const TileCtx c = tile_ctx<16>(a);  // pixel k = 0 of this lane: (lx, ly), ly in 0..3
   const unsigned lane = threadIdx.x & 63;
@@ -767,11 +774,11 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       for (int k = 0; k < DMGS_GACC_USED; ++k) s_acc[buf][s][k] = 0.f;
     }
     if constexpr (CELLS) {
-      n = stage_cell_batch<NT, true>(L, s_pos, s_wc, a, c, cursor, 0);
+      n = stage_cell_batch<NT, true>(L, s_wc, a, c, cursor, 0);
     } else {
       for (int s = threadIdx.x; s < BATCH; s += NT) {
         const int pos = top - 1 - s;
-        if (pos >= 0) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows);
+        if (pos >= 0) stage_splat(L, s, a.splats, a.point_list[c.range.x + pos], a.P, c.rows, (uint32_t)pos);
       }
       __syncthreads();
       n = min(BATCH, top);
@@ -783,18 +790,18 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     flush(buf ^ 1, prev_n);  // the previous batch's atomics overlap this batch
     __syncthreads();
-    uint32_t mask_next = n > 0 ? L.mask[0] : 0u;  // read one splat ahead
+    uint32_t mask_next = n > 0 ? splat_mask(L, 0) : 0u;  // read one splat ahead
     for (int j = 0; j < n; ++j) {
       const uint32_t reach = __builtin_amdgcn_readfirstlane(mask_next);
-      if (j + 1 < n) mask_next = L.mask[j + 1];
+      if (j + 1 < n) mask_next = splat_mask(L, j + 1);
       if (reach == 0u) continue;
       uint32_t contributor;
       if constexpr (CELLS) {
-        contributor = s_pos[j];
+        contributor = splat_pos(L, j);
       } else {
         contributor = (uint32_t)(top - 1 - j);
       }
-      const float2 xy = L.xy[j];
+      const float2 xy = make_float2(L.xym[j].x, L.xym[j].y);
       const float4 q = L.qo[j];
       const float4 col = L.rgb_id[j];
       const float dx = xy.x - pxf;
