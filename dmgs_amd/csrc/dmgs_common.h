@@ -155,13 +155,17 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
 
 // Exclusive scan of one value per thread over the work-group (s_wave: one
 // word per wave); total receives the sum.  Every thread must call it.
+// NW > 0: the work-group has exactly NW waves (a fixed-length wave loop).
+template <int NW = 0>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& total) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t inc = wave_inclusive_scan(v);
   if (lane == 63) s_wave[wave] = inc;
   __syncthreads();
   uint32_t wave_off = 0, tot = 0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+  const int nw = NW > 0 ? NW : (int)(blockDim.x >> 6);
+#pragma unroll
+  for (int w = 0; w < nw; ++w) {
     const uint32_t x = s_wave[w];
     if (w < wave) wave_off += x;
     tot += x;

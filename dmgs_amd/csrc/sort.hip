@@ -261,27 +261,46 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   __shared__ uint32_t wcnt[kSortWaves][256];
   __shared__ uint32_t dstart[256];  // block-local start of each digit
   __shared__ uint32_t gbase[256];   // global position of the block's first element of each digit
-  __shared__ uint32_t skey[kTile], sval[kTile];
+  // the block's keys, then its values, in block-sorted order: one buffer for
+  // both keeps the LDS per work-group small enough for seven per CU
+  __shared__ uint32_t sbuf[kTile];
+  __shared__ uint8_t sdig[kTile];  // digit of each block-sorted slot
   __shared__ uint32_t s_wave[kSortWaves];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: scalar addressing
   const uint32_t mask = (1u << nbits) - 1u;
   for (int w = 0; w < kSortWaves; ++w) wcnt[w][threadIdx.x] = 0;
-  __syncthreads();
+  // global position of the block's first element of each digit (one thread
+  // per digit), before any key is held in registers
+  const uint32_t dg = threadIdx.x;
+  {
+    uint32_t all_total;
+    const uint32_t digit_base = block_exclusive_scan<kSortWaves>(dg <= mask ? totals[dg] : 0u, s_wave, all_total);
+    if (dg <= mask) gbase[dg] = digit_base + hist_excl[(size_t)dg * nb + blockIdx.x];
+  }
 
   const size_t wbase = (size_t)blockIdx.x * kTile + (size_t)wave * kSlice;
   const uint64_t lt = lanemask_lt();
-  uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+  // the wave's slice through wave-uniform base pointers and 32-bit lane
+  // offsets: one address register for all loads instead of a 64-bit address each
+  const uint32_t* __restrict__ kw = kin + wbase;
+  const uint32_t* __restrict__ vw = vin + wbase;
+  const uint32_t rem = n > wbase ? (uint32_t)min(n - wbase, (size_t)kSlice) : 0u;
+  // keys now; the values only once the keys are written (fewer live registers)
+  uint32_t key[ITEMS], rank[ITEMS];
+  if (rem == (uint32_t)kSlice) {  // a full slice (all but the last): no per-item guards
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const size_t idx = wbase + (size_t)i * 64 + lane;
-    const bool valid = idx < n;
-    key[i] = valid ? kin[idx] : 0u;
-    val[i] = valid ? vin[idx] : 0u;
+    for (int i = 0; i < ITEMS; ++i) key[i] = kw[i * 64 + lane];
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t o = (uint32_t)(i * 64) + (uint32_t)lane;
+      key[i] = o < rem ? kw[o] : 0u;
+    }
   }
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const size_t idx = wbase + (size_t)i * 64 + lane;
-    const bool valid = idx < n;
+    const bool valid = (uint32_t)(i * 64) + (uint32_t)lane < rem;
     const uint32_t d = (key[i] >> shift) & mask;
     const uint64_t peers = digit_peers(d, nbits, valid);
     const uint32_t before = wcnt[wave][d];
@@ -290,9 +309,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     if (valid && lane == leader) wcnt[wave][d] = before + (uint32_t)__popcll(peers);
   }
   __syncthreads();
-  // per digit (one thread each): exclusive prefix over waves, block-local
-  // digit start, and the global position of the block's first element
-  const uint32_t dg = threadIdx.x;
+  // per digit (one thread each): exclusive prefix over waves and block-local
+  // digit start
   uint32_t tot = 0;
   if (dg <= mask) {
     for (int w = 0; w < kSortWaves; ++w) {
@@ -301,37 +319,48 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
       tot += c;
     }
   }
-  uint32_t block_total, all_total;
-  const uint32_t start = block_exclusive_scan(tot, s_wave, block_total);
-  const uint32_t digit_base = block_exclusive_scan(dg <= mask ? totals[dg] : 0u, s_wave, all_total);
-  dstart[dg] = start;
-  if (dg <= mask) gbase[dg] = digit_base + hist_excl[(size_t)dg * nb + blockIdx.x];
+  uint32_t block_total;
+  dstart[dg] = block_exclusive_scan<kSortWaves>(tot, s_wave, block_total);
   __syncthreads();
-  // stage the block's pairs in LDS in sorted order
+  // block-local sorted position of every item (kTile: none)
+  uint32_t lpos[ITEMS];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const size_t idx = wbase + (size_t)i * 64 + lane;
-    if (idx < n) {
+    lpos[i] = (uint32_t)kTile;
+    if ((uint32_t)(i * 64) + (uint32_t)lane < rem) {
       const uint32_t d = (key[i] >> shift) & mask;
-      const uint32_t lpos = dstart[d] + wcnt[wave][d] + rank[i];
-      if (lpos < (uint32_t)kTile) {
-        skey[lpos] = key[i];
-        sval[lpos] = val[i];
-      }
+      lpos[i] = dstart[d] + wcnt[wave][d] + rank[i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if (lpos[i] < (uint32_t)kTile) {
+      sbuf[lpos[i]] = key[i];
+      sdig[lpos[i]] = (uint8_t)((key[i] >> shift) & mask);
     }
   }
   __syncthreads();
-  // coalesced write-out: each digit's run is contiguous in LDS and in memory
+  // coalesced write-out, keys then values: each digit's run is contiguous in
+  // LDS and in memory
   const size_t bbase = (size_t)blockIdx.x * kTile;
   const uint32_t count = (uint32_t)min((size_t)kTile, n - bbase);
   for (uint32_t i = threadIdx.x; i < count; i += kSortThreads) {
-    const uint32_t k = skey[i];
-    const uint32_t d = (k >> shift) & mask;
+    const uint32_t d = sdig[i];
     const uint32_t pos = gbase[d] + (i - dstart[d]);
-    if (pos < n) {  // always true for a consistent histogram; guards the write
-      kout[pos] = k;
-      vout[pos] = sval[i];
-    }
+    if (pos < n) kout[pos] = sbuf[i];  // always true for a consistent histogram; guards the write
+  }
+  uint32_t val[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) val[i] = lpos[i] < (uint32_t)kTile ? vw[i * 64 + lane] : 0u;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i)
+    if (lpos[i] < (uint32_t)kTile) sbuf[lpos[i]] = val[i];
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < count; i += kSortThreads) {
+    const uint32_t d = sdig[i];
+    const uint32_t pos = gbase[d] + (i - dstart[d]);
+    if (pos < n) vout[pos] = sbuf[i];
   }
 }
 
