@@ -226,7 +226,7 @@ def run_plain(model, cams, targets, bg, steps, warmup):
             "stage_ms": {k: round(v, 4) for k, v in timer.averages_ms().items()}}
 
 
-def serving_run(n_requests, rate):
+def serving_run(n_requests, rate, sweep_rates=(100.0, 400.0, 1600.0)):
     """Four resident scenes (0.5M / 1M / 2M / 3M Gaussians, same
     distributions), Poisson arrivals at ``rate``, random scene, camera on a
     sphere of radius U[3, 6], resolution drawn from 256^2 .. 1920x1080;
@@ -241,9 +241,18 @@ def serving_run(n_requests, rate):
     names = sorted(scenes)
     srv = SV.RenderServer(scenes, max_views=8, num_streams=4)
     SV.serve_once(srv, SV.poisson_requests(names, 16, None, seed=99))  # warm-up
+    # warm-up at a moderate rate: the first batches of each (scene, size class) time both blend shapes
+    SV.serve_once(srv, SV.poisson_requests(names, 150, 400.0, seed=98))
     stats = SV.serve_once(srv, SV.poisson_requests(names, n_requests, rate, seed=3))
     stats.update(rate_per_s=rate, requests=n_requests, budget_GB=round(srv.budget / 1e9, 1),
                  peak_reserved_GB=round(srv.peak_reserved / 1e9, 3), max_in_flight=srv.max_active)
+    sweep = []
+    for r in sweep_rates:  # the same request mix at other Poisson rates
+        st = SV.serve_once(srv, SV.poisson_requests(names, n_requests, r, seed=int(r)))
+        sweep.append({k: st[k] for k in ("p50_ms", "p99_ms", "p50_ms_1080p", "p99_ms_1080p", "renders_per_s",
+                                         "mean_batch") if k in st})
+        sweep[-1]["rate_per_s"] = r
+    stats["sweep"] = sweep
     return stats
 
 

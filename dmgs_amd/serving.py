@@ -26,6 +26,7 @@ single-view result), so results do not depend on the mix.
 """
 import collections
 import itertools
+import statistics
 import time
 
 import numpy as np
@@ -73,7 +74,8 @@ class _Batch:
         with torch.cuda.stream(stream):
             self.r = BT.BatchedRenderer([q.camera for q in requests], [q.bg for q in requests], params.shape[1],
                                         sh_degree=sh, scale_modifier=requests[0].scaling_modifier,
-                                        device=server.device, variant=server.variant)
+                                        device=server.device)
+            self.r.variant, self.vkey = server._choose_variant(scene, self.r.total_tiles)
             self.r._alloc_state()
             self.images = self.r.new_images()
             self.count = torch.empty(2, dtype=torch.int32, pin_memory=True)  # instances, status word
@@ -81,6 +83,8 @@ class _Batch:
         self.K = None
         self.phase = "geometry"
         self.event = torch.cuda.Event()
+        self.t_start = torch.cuda.Event(enable_timing=True)
+        self.t_end = torch.cuda.Event(enable_timing=True)
 
     def fixed_bytes(self):
         return self.r.state.numel() + self.images.numel() * 4
@@ -98,9 +102,18 @@ class _Batch:
             stages = BT.STAGES_ALL
         with torch.cuda.stream(self.stream):
             self.r.plan(self.params, K_cap=max(self.K, 1), backward=False)
+            self.t_start.record(self.stream)
             self.r.forward(self.params, self.images, stages=stages)
+            self.t_end.record(self.stream)
             self.event.record(self.stream)
         self.phase = "render"
+
+
+# Blend shapes the server chooses between when no variant is fixed: quarter
+# tiles of 8x8 pixels (fastest for small views and short lists) and one wave
+# per 16x16 tile (fastest for large views and long lists).  All shapes give
+# bit-identical images, so the choice changes only the time.
+AUTO_VARIANTS = (3, 4)
 
 
 class RenderServer:
@@ -108,9 +121,12 @@ class RenderServer:
 
     scenes: {name: params (59, P) float32 on ``device``} or {name: (params,
     sh_degree)}.  hbm_budget: bytes the server may hold for in-flight work
-    (default 80 % of the free HBM at start-up)."""
+    (default 80 % of the free HBM at start-up).  variant: the blend shape of
+    every batch, or None to time each of AUTO_VARIANTS on the first batches
+    of a (scene, size class) and keep the faster (size class: the bit length
+    of the batch's tile count)."""
 
-    def __init__(self, scenes, device="cuda", hbm_budget=None, max_views=8, num_streams=4, variant=0,
+    def __init__(self, scenes, device="cuda", hbm_budget=None, max_views=8, num_streams=4, variant=None,
                  initial_instances_per_pixel=32.0):
         self.device = torch.device(device)
         if self.device.index is None:
@@ -122,7 +138,9 @@ class RenderServer:
                 raise ValueError(f"scene {name}: params must be a contiguous float32 (59, P) tensor")
             self.scenes[name] = (params.to(self.device), int(sh))
         self.budget = int(hbm_budget if hbm_budget is not None else 0.8 * torch.cuda.mem_get_info(self.device)[0])
-        self.max_views, self.variant = int(max_views), int(variant)
+        self.max_views = int(max_views)
+        self.variant = None if variant is None else int(variant)
+        self.variant_ms = {}  # (scene, size class) -> {variant: [render ms of its batches]}
         self.streams = [torch.cuda.Stream(self.device) for _ in range(num_streams)]
         self.free_streams = collections.deque(self.streams)
         self.pending = collections.deque()
@@ -151,6 +169,18 @@ class RenderServer:
         P = self.scenes[scene][0].shape[1]
         fixed, pixels = self._state_bytes(P, cams)
         return fixed + self._lists_bytes(self.ipp[scene] * pixels)
+
+    # ---- blend shape ------------------------------------------------------
+    def _choose_variant(self, scene, tiles):
+        """(variant, timing key or None) for a new batch."""
+        if self.variant is not None:
+            return self.variant, None
+        key = (scene, int(tiles).bit_length())
+        seen = self.variant_ms.setdefault(key, {v: [] for v in AUTO_VARIANTS})
+        for v in AUTO_VARIANTS:  # two samples of each first
+            if len(seen[v]) < 2:
+                return v, key
+        return min(AUTO_VARIANTS, key=lambda v: statistics.median(seen[v][-8:])), key
 
     # ---- request flow -----------------------------------------------------
     def submit(self, req):
@@ -238,6 +268,8 @@ class RenderServer:
 
     def _finish(self, b, error=None):
         now = time.perf_counter()
+        if error is None and b.vkey is not None and b.phase == "render":
+            self.variant_ms[b.vkey][b.r.variant].append(b.t_start.elapsed_time(b.t_end))
         for v, q in enumerate(b.requests):
             q.t_done = now
             if error is None:
