@@ -87,6 +87,9 @@ _SIGNATURES = {
                                                c_void_p, c_void_p, c_void_p]),
     "dmgs_dssim_l1_backward_groups": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32, c_float,
                                                 c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dmgs_dssim_target_stats": (c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_void_p]),
+    "dmgs_dssim_l1_forward_groups_stats": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                                     c_int32, c_float, c_void_p, c_void_p, c_void_p]),
     "dmgs_scan_scratch_size": (c_size_t, [c_int64]),
     "dmgs_inclusive_scan_u32": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
     "dmgs_batched_prepare_views": (c_int32, [ctypes.POINTER(DmgsView), c_int32, ctypes.POINTER(c_int32),

@@ -370,16 +370,24 @@ class BatchedLoss:
                 self.runs.append([v, v + 1, hw])
         self.ws = [torch.empty(self.lib.dmgs_dssim_l1_workspace_size(3 * (v1 - v0), h, w), dtype=torch.uint8,
                                device=dev) for v0, v1, (h, w) in self.runs]
+        # the targets' blurred maps (G*y, G*y^2) are fixed: computed once here
+        self.stats = []
+        s = N.stream_handle(dev)
+        for v0, v1, (h, w) in self.runs:
+            st = torch.empty(2 * 3 * (v1 - v0) * h * w, dtype=torch.float32, device=dev)
+            N.check(self.lib.dmgs_dssim_target_stats(N.ptr(renderer.image(self.targets, v0)), 3 * (v1 - v0), h, w,
+                                                     N.ptr(st), s), "dmgs_dssim_target_stats")
+            self.stats.append(st)
         self.losses = torch.zeros(renderer.B, 3, dtype=torch.float32, device=dev)
         self.ones = torch.ones(renderer.B, dtype=torch.float32, device=dev)
 
     def forward(self, images):
         """(B, 3) device tensor: (loss, l1, ssim) per view."""
         r, s = self.r, N.stream_handle(self.r.device)
-        for ws, (v0, v1, (h, w)) in zip(self.ws, self.runs):
-            N.check(self.lib.dmgs_dssim_l1_forward_groups(
-                N.ptr(r.image(images, v0)), N.ptr(r.image(self.targets, v0)), v1 - v0, 3, h, w, self.lam, N.ptr(ws),
-                N.ptr(self.losses[v0]), s), "dmgs_dssim_l1_forward_groups")
+        for ws, st, (v0, v1, (h, w)) in zip(self.ws, self.stats, self.runs):
+            N.check(self.lib.dmgs_dssim_l1_forward_groups_stats(
+                N.ptr(r.image(images, v0)), N.ptr(r.image(self.targets, v0)), N.ptr(st), v1 - v0, 3, h, w, self.lam,
+                N.ptr(ws), N.ptr(self.losses[v0]), s), "dmgs_dssim_l1_forward_groups_stats")
         return self.losses
 
     def backward(self, images, dimages):

@@ -510,8 +510,20 @@ size_t dmgs_dssim_l1_workspace_size(int32_t C, int32_t H, int32_t W) {
   return (C <= 0 || H <= 0 || W <= 0) ? 0 : SsimLayout(C, H, W).total;
 }
 
-int dmgs_dssim_l1_forward_groups(const float* img, const float* gt, int32_t G, int32_t Cg, int32_t H, int32_t W,
-                                 float lambda_dssim, void* ws, float* loss_out, dmgs_stream_t stream) {
+int dmgs_dssim_target_stats(const float* gt, int32_t C, int32_t H, int32_t W, float* stats, dmgs_stream_t stream) {
+  if (!gt || !stats) return fail(DMGS_EINVAL, "null pointer");
+  if (C <= 0 || H <= 0 || W <= 0 || (int64_t)C * H * W > ((int64_t)1 << 31))
+    return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d", C, H, W);
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  float w[11];
+  ssim_window(w);
+  dmgs::launch_dssim_target(gt, C, H, W, w, stats, st);
+  return check_launch("dssim target statistics", st, false);
+}
+
+int dmgs_dssim_l1_forward_groups_stats(const float* img, const float* gt, const float* gt_stats, int32_t G,
+                                       int32_t Cg, int32_t H, int32_t W, float lambda_dssim, void* ws,
+                                       float* loss_out, dmgs_stream_t stream) {
   if (!img || !gt || !ws || !loss_out) return fail(DMGS_EINVAL, "null pointer");
   if (G <= 0 || Cg <= 0 || H <= 0 || W <= 0 || (int64_t)G * Cg * H * W > ((int64_t)1 << 31))
     return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d x %d", G, Cg, H, W);
@@ -521,9 +533,14 @@ int dmgs_dssim_l1_forward_groups(const float* img, const float* gt, int32_t G, i
   float w[11];
   ssim_window(w);
   // the final reduction writes (loss, l1, ssim) per group straight to loss_out (no copy node when captured)
-  dmgs::launch_dssim_forward(img, gt, G * Cg, Cg, H, W, w, lambda_dssim, at<float>(ws, L.coef),
+  dmgs::launch_dssim_forward(img, gt, gt_stats, G * Cg, Cg, H, W, w, lambda_dssim, at<float>(ws, L.coef),
                              at<float>(ws, L.partials), loss_out, st);
   return check_launch("dssim forward", st, false);
+}
+
+int dmgs_dssim_l1_forward_groups(const float* img, const float* gt, int32_t G, int32_t Cg, int32_t H, int32_t W,
+                                 float lambda_dssim, void* ws, float* loss_out, dmgs_stream_t stream) {
+  return dmgs_dssim_l1_forward_groups_stats(img, gt, nullptr, G, Cg, H, W, lambda_dssim, ws, loss_out, stream);
 }
 
 int dmgs_dssim_l1_backward_groups(const float* img, const float* gt, int32_t G, int32_t Cg, int32_t H, int32_t W,

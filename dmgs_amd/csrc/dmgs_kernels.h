@@ -270,8 +270,11 @@ void launch_faces_in_frustum(int F, const float* verts, const int32_t* faces, co
 // ---- fused L1 + SSIM loss ----
 size_t dssim_partials(int C, int H, int W);
 // C planes in C / group loss groups of `group` consecutive planes each.
-void launch_dssim_forward(const float* x, const float* y, int C, int group, int H, int W, const float* w11,
-                          float lam, float* coef, float* partials, float* out, hipStream_t s);
+// ystats: the target's blurred maps from launch_dssim_target, or null.
+void launch_dssim_forward(const float* x, const float* y, const float* ystats, int C, int group, int H, int W,
+                          const float* w11, float lam, float* coef, float* partials, float* out, hipStream_t s);
+// stats (2 C planes): G*y, then G*y^2.
+void launch_dssim_target(const float* y, int C, int H, int W, const float* w11, float* stats, hipStream_t s);
 void launch_dssim_backward(const float* x, const float* y, const float* coef, int C, int group, int H, int W,
                            const float* w11, float lam, const float* dloss, float* dx, hipStream_t s);
 

@@ -18,6 +18,11 @@
 //   a = dS/dmu1, b = dS/d(E[x^2]), c = dS/d(E[xy])   (each times dL/dS).
 // Backward: dL/dx = G*a + 2 x (G*b) + y (G*c) + (1 - lambda) sign(x - y) / N,
 // with the same tiling over the three coefficient planes.
+//
+// Fixed targets: the two target maps G*y and G*y^2 do not change between
+// steps; dssim_target_kernel computes them once (the forward's own loops, so
+// the values are bit-identical) and the forward then blurs three maps
+// (x, x^2, xy) instead of five.
 #include "dmgs_common.h"
 #include "dmgs_kernels.h"
 
@@ -78,12 +83,16 @@ __device__ __forceinline__ void load14(const float (*src)[kPS], int r, int j, fl
 
 // One loss per group of `group` consecutive planes (group = C: one loss over
 // the whole image; group = 3: one per RGB view of a batch).
+// STATS: ystats holds G*y (planes 0..C-1) and G*y^2 (planes C..2C-1).
+template <bool STATS>
 __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                        int H, int W, int group, Window11 g, float lam,
-                                                        float* __restrict__ coef, float* __restrict__ partials) {
+                                                        const float* __restrict__ ystats, int H, int W, int group,
+                                                        Window11 g, float lam, float* __restrict__ coef,
+                                                        float* __restrict__ partials) {
+  constexpr int NQ = STATS ? 3 : 5;  // blurred maps: x, x^2, xy (, y, y^2)
   __shared__ __align__(16) float sx[kPH][kPS];
   __shared__ __align__(16) float sy[kPH][kPS];
-  __shared__ __align__(16) float hq[5][kPH][kHS];
+  __shared__ __align__(16) float hq[NQ][kPH][kHS];
   __shared__ float red[2][4];
   const int c = blockIdx.z;
   const int ox = blockIdx.x * kTW, oy = blockIdx.y * kTH;
@@ -111,17 +120,22 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
         if (k < 0 || k > 10) continue;
         const float w = g.w[k];
         m1[i] = fmaf(w, a[e], m1[i]);
-        m2[i] = fmaf(w, b[e], m2[i]);
         s11[i] = fmaf(w, aa, s11[i]);
-        s22[i] = fmaf(w, bb, s22[i]);
         s12[i] = fmaf(w, ab, s12[i]);
+        if constexpr (!STATS) {
+          m2[i] = fmaf(w, b[e], m2[i]);
+          s22[i] = fmaf(w, bb, s22[i]);
+        }
       }
     }
+    // map order: x, x^2, xy, then y, y^2
     *reinterpret_cast<float4*>(&hq[0][r][4 * j]) = make_float4(m1[0], m1[1], m1[2], m1[3]);
-    *reinterpret_cast<float4*>(&hq[1][r][4 * j]) = make_float4(m2[0], m2[1], m2[2], m2[3]);
-    *reinterpret_cast<float4*>(&hq[2][r][4 * j]) = make_float4(s11[0], s11[1], s11[2], s11[3]);
-    *reinterpret_cast<float4*>(&hq[3][r][4 * j]) = make_float4(s22[0], s22[1], s22[2], s22[3]);
-    *reinterpret_cast<float4*>(&hq[4][r][4 * j]) = make_float4(s12[0], s12[1], s12[2], s12[3]);
+    *reinterpret_cast<float4*>(&hq[1][r][4 * j]) = make_float4(s11[0], s11[1], s11[2], s11[3]);
+    *reinterpret_cast<float4*>(&hq[2][r][4 * j]) = make_float4(s12[0], s12[1], s12[2], s12[3]);
+    if constexpr (!STATS) {
+      *reinterpret_cast<float4*>(&hq[3][r][4 * j]) = make_float4(m2[0], m2[1], m2[2], m2[3]);
+      *reinterpret_cast<float4*>(&hq[4][r][4 * j]) = make_float4(s22[0], s22[1], s22[2], s22[3]);
+    }
   }
   __syncthreads();
   // vertical: four outputs of one column per thread
@@ -134,7 +148,7 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
 #pragma unroll
   for (int e = 0; e < 14; ++e) {
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       const float h = hq[q][r0 + e][lx];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -144,18 +158,28 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
       }
     }
   }
+  const int px = ox + lx;
+  if constexpr (STATS) {  // the target's maps, computed once by dssim_target_kernel
+    const size_t ally = (size_t)gridDim.z * plane;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int py = oy + r0 + i;
+      const size_t o = c * plane + (size_t)min(py, H - 1) * W + min(px, W - 1);
+      v[i][3] = ystats[o];
+      v[i][4] = ystats[ally + o];
+    }
+  }
   const float C1 = 0.01f * 0.01f, C2 = 0.03f * 0.03f;
   const float dS = -lam / (float)((size_t)group * plane);  // dL/dS for the mean over the group's pixels
   const size_t all = (size_t)gridDim.z * plane;
-  const int px = ox + lx;
   float s_acc = 0.f, l_acc = 0.f;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int py = oy + r0 + i;
     if (px >= W || py >= H) continue;
-    const float mu1 = v[i][0], mu2 = v[i][1];
+    const float mu1 = v[i][0], mu2 = v[i][3];
     const float mu1_sq = mu1 * mu1, mu2_sq = mu2 * mu2, mu12 = mu1 * mu2;
-    const float sig1 = v[i][2] - mu1_sq, sig2 = v[i][3] - mu2_sq, sig12 = v[i][4] - mu12;
+    const float sig1 = v[i][1] - mu1_sq, sig2 = v[i][4] - mu2_sq, sig12 = v[i][2] - mu12;
     const float A1 = 2.f * mu12 + C1, A2 = 2.f * sig12 + C2;
     const float B1 = mu1_sq + mu2_sq + C1, B2 = sig1 + sig2 + C2;
     const float inv = 1.f / (B1 * B2);
@@ -183,6 +207,72 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
     const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     partials[2 * b] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
     partials[2 * b + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+// The target's maps G*y and G*y^2 (stats planes 0..C-1 and C..2C-1), with
+// the forward's tiling and loops.
+__global__ __launch_bounds__(256) void dssim_target_kernel(const float* __restrict__ y, int H, int W, Window11 g,
+                                                           float* __restrict__ stats) {
+  __shared__ __align__(16) float sy[kPH][kPS];
+  __shared__ __align__(16) float hq[2][kPH][kHS];
+  const int c = blockIdx.z;
+  const int ox = blockIdx.x * kTW, oy = blockIdx.y * kTH;
+  const size_t plane = (size_t)H * W;
+  {
+    float(*const dst[1])[kPS] = {sy};
+    const float* const src[1] = {y + c * plane};
+    stage_patches<1>(dst, src, H, W, ox, oy);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kHTasks; t += 256) {
+    const int r = t / kQuads, j = t - r * kQuads;
+    float b[14];
+    load14(sy, r, j, b);
+    float m2[4] = {0.f, 0.f, 0.f, 0.f}, s22[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int e = 0; e < 14; ++e) {
+      const float bb = b[e] * b[e];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = e - i;
+        if (k < 0 || k > 10) continue;
+        const float w = g.w[k];
+        m2[i] = fmaf(w, b[e], m2[i]);
+        s22[i] = fmaf(w, bb, s22[i]);
+      }
+    }
+    *reinterpret_cast<float4*>(&hq[0][r][4 * j]) = make_float4(m2[0], m2[1], m2[2], m2[3]);
+    *reinterpret_cast<float4*>(&hq[1][r][4 * j]) = make_float4(s22[0], s22[1], s22[2], s22[3]);
+  }
+  __syncthreads();
+  const int lx = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
+  float v[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v[i][0] = v[i][1] = 0.f;
+#pragma unroll
+  for (int e = 0; e < 14; ++e) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const float h = hq[q][r0 + e][lx];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = e - i;
+        if (k < 0 || k > 10) continue;
+        v[i][q] = fmaf(g.w[k], h, v[i][q]);
+      }
+    }
+  }
+  const int px = ox + lx;
+  if (px >= W) return;
+  const size_t all = (size_t)gridDim.z * plane;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int py = oy + r0 + i;
+    if (py >= H) break;
+    const size_t o = c * plane + (size_t)py * W + px;
+    stats[o] = v[i][0];
+    stats[all + o] = v[i][1];
   }
 }
 
@@ -292,12 +382,24 @@ __global__ __launch_bounds__(256) void dssim_bwd_kernel(const float* __restrict_
   }
 }
 
-void launch_dssim_forward(const float* x, const float* y, int C, int group, int H, int W, const float* w11,
-                          float lam, float* coef, float* partials, float* out, hipStream_t s) {
+void launch_dssim_target(const float* y, int C, int H, int W, const float* w11, float* stats, hipStream_t s) {
   Window11 g;
   for (int k = 0; k < 11; ++k) g.w[k] = w11[k];
   const dim3 grid((W + kTW - 1) / kTW, (H + kTH - 1) / kTH, C);
-  hipLaunchKernelGGL(dssim_fwd_kernel, grid, dim3(256), 0, s, x, y, H, W, group, g, lam, coef, partials);
+  hipLaunchKernelGGL(dssim_target_kernel, grid, dim3(256), 0, s, y, H, W, g, stats);
+}
+
+void launch_dssim_forward(const float* x, const float* y, const float* ystats, int C, int group, int H, int W,
+                          const float* w11, float lam, float* coef, float* partials, float* out, hipStream_t s) {
+  Window11 g;
+  for (int k = 0; k < 11; ++k) g.w[k] = w11[k];
+  const dim3 grid((W + kTW - 1) / kTW, (H + kTH - 1) / kTH, C);
+  if (ystats)
+    hipLaunchKernelGGL(dssim_fwd_kernel<true>, grid, dim3(256), 0, s, x, y, ystats, H, W, group, g, lam, coef,
+                       partials);
+  else
+    hipLaunchKernelGGL(dssim_fwd_kernel<false>, grid, dim3(256), 0, s, x, y, ystats, H, W, group, g, lam, coef,
+                       partials);
   const int per_group = (int)(grid.x * grid.y) * group;
   hipLaunchKernelGGL(dssim_reduce_kernel, dim3(C / group), dim3(256), 0, s, (const float*)partials, per_group,
                      1.0f / (float)((size_t)group * H * W), lam, out);

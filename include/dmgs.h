@@ -212,6 +212,17 @@ int dmgs_dssim_l1_forward_groups(const float* img, const float* gt, int32_t G, i
 int dmgs_dssim_l1_backward_groups(const float* img, const float* gt, int32_t G, int32_t Cg, int32_t H, int32_t W,
                                   float lambda_dssim, const void* ws, const float* dloss, float* dimg,
                                   dmgs_stream_t stream);
+/* Fixed targets: stats (2*C planes of H x W, float32) receives the target's
+ * Gaussian-blurred maps G*gt (planes 0..C-1) and G*gt^2 (planes C..2C-1), the
+ * two SSIM inputs that do not depend on the rendered image.  The forward
+ * below then blurs three maps per plane instead of five; its results are
+ * bit-identical to dmgs_dssim_l1_forward_groups' (the same loops compute the
+ * maps).  No reference counterpart: utils/loss_utils.py:37-62 recomputes them
+ * on every call. */
+int dmgs_dssim_target_stats(const float* gt, int32_t C, int32_t H, int32_t W, float* stats, dmgs_stream_t stream);
+int dmgs_dssim_l1_forward_groups_stats(const float* img, const float* gt, const float* gt_stats, int32_t G,
+                                       int32_t Cg, int32_t H, int32_t W, float lambda_dssim, void* ws,
+                                       float* loss_out, dmgs_stream_t stream);
 
 /* ---- batched views --------------------------------------------------------
  * Replaces the per-view loop of the training / evaluation scripts

@@ -71,3 +71,48 @@ def test_grouped_loss_equals_per_image_calls():
         torch.cuda.synchronize()
         assert torch.allclose(out[k], o1, rtol=1e-6, atol=1e-7), (k, out[k], o1)
         assert torch.allclose(dimg[3 * k:3 * k + 3], d1, rtol=1e-6, atol=1e-12), k
+
+
+@pytest.mark.parametrize("G,H,W", [(4, 70, 90), (2, 33, 200), (1, 800, 800)])
+def test_fixed_target_statistics_give_identical_loss(G, H, W):
+    """The forward on precomputed target maps (G*y, G*y^2) gives bit-identical
+    losses and backward coefficients to the forward that blurs all five maps,
+    and the maps themselves are the 11x11 Gaussian blur of y and y^2."""
+    from dmgs_amd import _native as N
+    print(f"case: fixed-target statistics, G={G} 3x{H}x{W}", flush=True)
+    lib = N.load()
+    g = torch.Generator().manual_seed(21 + H)
+    img = torch.rand(G * 3, H, W, generator=g).cuda()
+    gt = torch.rand(G * 3, H, W, generator=g).cuda()
+    dloss = torch.linspace(0.5, 1.5, G).cuda()
+    s = N.stream_handle()
+    stats = torch.empty(2, G * 3, H, W, device="cuda")
+    N.check(lib.dmgs_dssim_target_stats(N.ptr(gt), G * 3, H, W, N.ptr(stats), s), "stats")
+    outs, grads = [], []
+    for use in (False, True):
+        ws = torch.empty(lib.dmgs_dssim_l1_workspace_size(G * 3, H, W), dtype=torch.uint8, device="cuda")
+        out = torch.empty(G, 3, device="cuda")
+        dimg = torch.empty_like(img)
+        if use:
+            N.check(lib.dmgs_dssim_l1_forward_groups_stats(N.ptr(img), N.ptr(gt), N.ptr(stats), G, 3, H, W, 0.2,
+                                                           N.ptr(ws), N.ptr(out), s), "f stats")
+        else:
+            N.check(lib.dmgs_dssim_l1_forward_groups(N.ptr(img), N.ptr(gt), G, 3, H, W, 0.2, N.ptr(ws), N.ptr(out),
+                                                     s), "f")
+        N.check(lib.dmgs_dssim_l1_backward_groups(N.ptr(img), N.ptr(gt), G, 3, H, W, 0.2, N.ptr(ws), N.ptr(dloss),
+                                                  N.ptr(dimg), s), "b")
+        outs.append(out)
+        grads.append(dimg)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+    assert torch.equal(grads[0], grads[1])
+    # the maps: depthwise conv2d with the normalised 11-tap Gaussian (sigma 1.5), zero padding 5
+    x = torch.arange(11, dtype=torch.float64) - 5
+    w1 = torch.exp(-x * x / (2 * 1.5 ** 2))
+    w1 = w1 / w1.sum()
+    win = (w1[:, None] * w1[None, :]).expand(G * 3, 1, 11, 11).contiguous()
+    y64 = gt.double().cpu()[None]
+    ref_mu = torch.nn.functional.conv2d(y64, win, padding=5, groups=G * 3)[0]
+    ref_sq = torch.nn.functional.conv2d(y64 * y64, win, padding=5, groups=G * 3)[0]
+    assert torch.allclose(stats[0].double().cpu(), ref_mu, atol=2e-6)
+    assert torch.allclose(stats[1].double().cpu(), ref_sq, atol=2e-6)
