@@ -251,7 +251,7 @@ size_t dmgs_batched_lists_size(int64_t capacity) {
 
 size_t dmgs_batched_backward_size(int32_t B, int64_t P) {
   if (B <= 0 || P <= 0) return 0;
-  return (size_t)B * P * DMGS_GACC_STRIDE * 4;
+  return align_up((size_t)B * P * DMGS_GACC_STRIDE * 4);  // whole 16-byte vectors for the forward's clear
 }
 
 int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels, int64_t capacity,
@@ -369,7 +369,7 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     r.out_color = images;
     if (b->bwd_ws) {  // the blend work-groups also clear the backward accumulator
       r.clear = static_cast<float4*>(b->bwd_ws);
-      r.clear_n = c.N * DMGS_GACC_STRIDE / 4;
+      r.clear_n = (c.N * DMGS_GACC_STRIDE + 3) / 4;  // inside the 256-byte-aligned workspace
     }
     dmgs::launch_render_fwd(r, st, b->variant);
     if ((rc = bcheck("batch render forward", st))) return rc;

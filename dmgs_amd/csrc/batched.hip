@@ -166,10 +166,11 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
     const int radius = a.radii[i];
     if (radius <= 0) continue;
     any = true;
-    float* gr = a.gacc + i * DMGS_GACC_STRIDE;
-    const float4 g03 = *reinterpret_cast<const float4*>(gr);
-    const float4 g47 = *reinterpret_cast<const float4*>(gr + 4);
-    const float g8 = gr[8];
+    const float3* gr = reinterpret_cast<const float3*>(a.gacc + i * DMGS_GACC_STRIDE);
+    const float3 r0 = gr[0], r1 = gr[1], r2 = gr[2];
+    const float4 g03 = make_float4(r0.x, r0.y, r0.z, r1.x);
+    const float4 g47 = make_float4(r1.y, r1.z, r2.x, r2.y);
+    const float g8 = r2.z;
     const ViewParams vp = view_params(const_ptr(a.views)[__builtin_amdgcn_readfirstlane(v)]);
     if (a.dens) {
       d_acc += sqrtf(g03.x * g03.x + g03.y * g03.y);

@@ -25,10 +25,12 @@
 #define DMGS_SPLAT_F4 3
 
 // Per-Gaussian accumulator written by the blend backward with atomics:
-// 12 floats (48 B: three 16-byte vectors, 9 used) per Gaussian.
+// 9 floats (36 B, read as three 12-byte vectors) per Gaussian, packed (the
+// rows are read and cleared as whole cache lines, so padding would cost
+// traffic).
 //   [0] dL/dmean2D.x (NDC)  [1] dL/dmean2D.y  [2] dL/dconic.a  [3] dL/dconic.b (half)
 //   [4] dL/dconic.c  [5] dL/dopacity  [6..8] dL/dcolour rgb
-#define DMGS_GACC_STRIDE 12
+#define DMGS_GACC_STRIDE 9
 #define DMGS_GACC_USED 9
 
 namespace dmgs {

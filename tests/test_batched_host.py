@@ -71,7 +71,8 @@ def test_workspace_and_offsets():
     # at least every compulsory buffer: 48 B splat + 7 x 4 B per entry, 8 B range per tile, 8 B per pixel
     assert n >= B * P * (48 + 28) + T * 8 + X * 8
     assert lib.dmgs_batched_lists_size(K) >= 16 * K
-    assert lib.dmgs_batched_backward_size(B, P) == B * P * 48
+    # one packed 36-byte accumulator row per (view, Gaussian), whole 256-byte blocks
+    assert lib.dmgs_batched_backward_size(B, P) == (B * P * 36 + 255) // 256 * 256
     assert lib.dmgs_batched_state_size(0, P, T, X) == 0
     assert lib.dmgs_batched_lists_size(0) == 0
     offs = (N.c_size_t * 9)()

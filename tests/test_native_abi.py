@@ -71,7 +71,7 @@ def test_workspace_sizes():
     assert lib.dmgs_binning_workspace_size(0) > 0
     assert lib.dmgs_binning_workspace_size(10_000_000) >= 16 * 10_000_000
     assert lib.dmgs_image_workspace_size(800, 800) >= 8 * 800 * 800
-    assert lib.dmgs_backward_workspace_size(10) >= 10 * 48  # one 48-byte accumulator row per Gaussian
+    assert lib.dmgs_backward_workspace_size(10) >= 10 * 36  # one 36-byte accumulator row per Gaussian
     offs = (ctypes.c_size_t * 10)()
     assert lib.dmgs_workspace_offsets(100, 1000, 64, 48, offs) == 0
     assert offs[0] == 0 and all(o % 256 == 0 for o in offs)
