@@ -116,6 +116,12 @@ __device__ __forceinline__ uint32_t view_major_key(uint32_t depth_bits, int view
   return ((uint32_t)view << 28) | (e << 24) | (depth_bits & 0xFFFFFFu);
 }
 
+// Up to kCompactViews views: every radius, every wave offset and every
+// visible entry's depth key is requested before the first store (the stores
+// could alias the loads as far as the compiler knows, so a per-view loop waits
+// for one memory round trip per view).
+constexpr int kCompactViews = 16;
+
 __global__ __launch_bounds__(256) void mv_compact_kernel(MVArgs a, const uint32_t* __restrict__ depth_keys,
                                                          uint32_t* __restrict__ keys_out,
                                                          uint32_t* __restrict__ ids_out, int view_major,
@@ -123,22 +129,46 @@ __global__ __launch_bounds__(256) void mv_compact_kernel(MVArgs a, const uint32_
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = g < a.P;
   const uint64_t lt = lanemask_lt();
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  auto wave_base = [&](int v) -> uint32_t {  // visible entries of all earlier waves (inclusive scan)
+    const size_t w = ((size_t)v * gridDim.x + blockIdx.x) * 4 + wave;  // 256-lane work-groups
+    return w == 0 ? 0u : a.wave_counts[w - 1];
+  };
+  auto emit = [&](int v, bool visible, uint32_t key, uint32_t base, uint64_t bal) {
+    if (!visible) return;
+    const size_t i = (size_t)v * a.P + g;
+    const uint32_t slot = base + (uint32_t)__popcll(bal & lt);
+    if (view_major) {
+      bool clamped;
+      key = view_major_key(key, v, clamped);
+      if (clamped) atomicOr(status, 1u);
+    }
+    keys_out[slot] = key;
+    ids_out[slot] = (uint32_t)i;
+  };
+  if (a.B <= kCompactViews) {
+    int rad[kCompactViews];
+    uint32_t base[kCompactViews], key[kCompactViews];
+#pragma unroll
+    for (int v = 0; v < kCompactViews; ++v) {
+      rad[v] = (v < a.B && live) ? a.radii[(size_t)v * a.P + g] : 0;
+      base[v] = v < a.B ? wave_base(v) : 0u;
+    }
+#pragma unroll
+    for (int v = 0; v < kCompactViews; ++v) key[v] = rad[v] > 0 ? depth_keys[(size_t)v * a.P + g] : 0u;
+#pragma unroll
+    for (int v = 0; v < kCompactViews; ++v) {
+      if (v >= a.B) break;
+      const bool visible = rad[v] > 0;
+      emit(v, visible, key[v], base[v], __ballot(visible));
+    }
+    return;
+  }
   for (int v = 0; v < a.B; ++v) {
     const size_t i = (size_t)v * a.P + g;
     const bool visible = live && a.radii[i] > 0;
     const uint64_t bal = __ballot(visible);
-    const size_t w = ((size_t)v * gridDim.x + blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (visible) {
-      const uint32_t slot = (w == 0 ? 0u : a.wave_counts[w - 1]) + (uint32_t)__popcll(bal & lt);
-      uint32_t key = depth_keys[i];
-      if (view_major) {
-        bool clamped;
-        key = view_major_key(key, v, clamped);
-        if (clamped) atomicOr(status, 1u);
-      }
-      keys_out[slot] = key;
-      ids_out[slot] = (uint32_t)i;
-    }
+    emit(v, visible, visible ? depth_keys[i] : 0u, wave_base(v), bal);
   }
 }
 
