@@ -60,7 +60,7 @@ __device__ __forceinline__ void load_mv(const MVArgs& a, int g, MVGaussian& G) {
 __device__ __forceinline__ void count_visible(const MVArgs& a, int v, bool visible) {
   const uint64_t bal = __ballot(visible);
   if ((threadIdx.x & 63) == 0)
-    a.wave_counts[((size_t)v * gridDim.x + blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6)] =
+    a.wave_counts[((size_t)v * gridDim.x + blockIdx.x) * 4 + (threadIdx.x >> 6)] =  // 256-lane work-groups
         (uint32_t)__popcll(bal);
 }
 

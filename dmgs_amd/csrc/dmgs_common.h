@@ -145,13 +145,17 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // Inclusive prefix sum across the wave (uint32).
+// Inclusive scan over the wave with DPP: prefix sums within each 16-lane row
+// (row_shr 1, 2, 4, 8; lanes whose source falls outside the row add 0), then
+// row_bcast:15 adds row r's last lane into row r + 1 (rows 1 and 3) and
+// row_bcast:31 adds lane 31 into rows 2 and 3.  Ten VALU, no LDS.
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-  const unsigned l = __lane_id();
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t o = __shfl_up(v, off, 64);
-    if (l >= (unsigned)off) v += o;
-  }
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
   return v;
 }
 

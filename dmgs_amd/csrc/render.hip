@@ -925,7 +925,7 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const uint8_t* __restri
   const int t = blockIdx.x * 256 + threadIdx.x;
   const uint32_t h = hist[threadIdx.x];
   uint32_t total;
-  start[threadIdx.x] = block_exclusive_scan(h, s_wave, total);
+  start[threadIdx.x] = block_exclusive_scan<4>(h, s_wave, total);  // 256-lane work-group
   cnt[threadIdx.x] = 0u;
   __syncthreads();
   uint32_t b = 0, rank = 0;

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
     if (idx < n) sum += in[idx];
   }
   uint32_t total;
-  block_exclusive_scan(sum, s_wave, total);
+  block_exclusive_scan<kScanThreads / 64>(sum, s_wave, total);
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_spine_kernel(uint32_t* __re
       tsum += v[i];
     }
     uint32_t total;
-    uint32_t run = carry + block_exclusive_scan(tsum, s_wave, total);
+    uint32_t run = carry + block_exclusive_scan<kScanThreads / 64>(tsum, s_wave, total);
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
       const size_t idx = base + (size_t)threadIdx.x * kScanItems + i;
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
     tsum += v[i];
   }
   uint32_t total;
-  uint32_t run = spine[blockIdx.x] + block_exclusive_scan(tsum, s_wave, total);  // ends with a barrier
+  uint32_t run = spine[blockIdx.x] + block_exclusive_scan<kScanThreads / 64>(tsum, s_wave, total);  // ends with a barrier
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
     const uint32_t x = v[i];
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kScanThreads) void radix_rowscan_kernel(uint32_t* _
       tsum += v[i];
     }
     uint32_t total;
-    uint32_t run = carry + block_exclusive_scan(tsum, s_wave, total);
+    uint32_t run = carry + block_exclusive_scan<kScanThreads / 64>(tsum, s_wave, total);
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
       const uint32_t idx = base + threadIdx.x * kScanItems + i;
