@@ -442,6 +442,19 @@ def main():
         torch.cuda.synchronize()
         blat.append(1000.0 * (time.perf_counter() - a))
     plat = plain_latency(model, cams[0], bg, args.render_samples)
+    # one view through the batched engine (resident packed parameters), forward only
+    r1 = BT.BatchedRenderer(cams[:1], bg, model.num, sh_degree=3, variant=3)
+    r1.plan(params, backward=False)
+    img1 = r1.new_images()
+    vlat = []
+    for k in range(3 + args.render_samples):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        r1.forward(params, img1)
+        torch.cuda.synchronize()
+        if k >= 3:
+            vlat.append(1000.0 * (time.perf_counter() - a))
+    del r1, img1
 
     plain = None
     if not args.no_plain:
@@ -480,6 +493,8 @@ def main():
         "render_ms_p99": round(float(np.percentile(plat, 99)), 3),
         "batch_render_ms_p50": round(float(np.percentile(blat, 50)), 3),
         "batch_render_ms_p99": round(float(np.percentile(blat, 99)), 3),
+        "view_render_ms_p50": round(float(np.percentile(vlat, 50)), 3),
+        "view_render_ms_p99": round(float(np.percentile(vlat, 99)), 3),
         "step_algorithmic_GBs": round(step_bytes / (ms_step * 1e-3) / 1e9, 2),
     }
     log(f"batched {it_s:.2f} it/s; plain {plain['value'] if plain else 'skipped'}; stages {stage_ms}")

@@ -45,13 +45,31 @@ class FreeGaussians:
     def num(self):
         return self._xyz.shape[0]
 
+    # Renders of an unchanged scene without autograd (viewers, serving,
+    # evaluation) reuse the activated tensors: a getter called under no_grad
+    # returns the value it computed last for the same parameter tensors at the
+    # same version (any in-place update or replacement of a parameter
+    # recomputes it).  With autograd on, every call builds its own graph, as
+    # the reference getters do.
+    def _memo(self, name, fn, *params):
+        if torch.is_grad_enabled():
+            return fn()
+        key = tuple((p.data_ptr(), p._version, tuple(p.shape)) for p in params)
+        memo = self.__dict__.setdefault("_memo_cache", {})
+        hit = memo.get(name)
+        if hit is not None and hit[0] == key:
+            return hit[1]
+        value = fn()
+        memo[name] = (key, value)
+        return value
+
     @property
     def get_scaling(self):
-        return torch.exp(self._scaling)
+        return self._memo("scaling", lambda: torch.exp(self._scaling), self._scaling)
 
     @property
     def get_rotation(self):
-        return torch.nn.functional.normalize(self._rotation)
+        return self._memo("rotation", lambda: torch.nn.functional.normalize(self._rotation), self._rotation)
 
     @property
     def get_xyz(self):
@@ -59,11 +77,12 @@ class FreeGaussians:
 
     @property
     def get_features(self):
-        return torch.cat((self._features_dc, self._features_rest), dim=1)
+        return self._memo("features", lambda: torch.cat((self._features_dc, self._features_rest), dim=1),
+                          self._features_dc, self._features_rest)
 
     @property
     def get_opacity(self):
-        return torch.sigmoid(self._opacity)
+        return self._memo("opacity", lambda: torch.sigmoid(self._opacity), self._opacity)
 
     # ---- densification statistics (the consumer of radii and the NDC
     # means2D gradient; scene/gaussian_model.py:405-407, train.py:116-118) ----

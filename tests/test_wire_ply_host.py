@@ -83,3 +83,24 @@ def test_ply_header_round_trip_and_refusals():
     bad = ply.attribute_names(3)[:-5] + ["opacity"]  # 44 rest coefficients
     with pytest.raises(ValueError):
         ply._sh_rest_per_colour([n for n in bad if n != "f_rest_44"])
+
+
+def test_scene_getters_reuse_values_only_without_autograd():
+    """FreeGaussians getters under no_grad return the previous value while the
+    parameters are unchanged and recompute after an in-place update; with
+    autograd on they always build a fresh graph."""
+    from dmgs_amd import scenes
+    m = scenes.free_scene(64, seed=3)
+    with torch.no_grad():
+        a = m.get_features
+        assert m.get_features is a
+        s0 = m.get_scaling
+        m._scaling.add_(1.0)  # in-place update: new version
+        s1 = m.get_scaling
+        assert s1 is not s0 and torch.allclose(s1, torch.exp(m._scaling))
+        m._features_dc = m._features_dc.clone()  # replaced parameter
+        assert m.get_features is not a
+    m.requires_grad_(True)
+    f1, f2 = m.get_features, m.get_features
+    assert f1 is not f2 and f1.requires_grad
+    assert torch.equal(m.get_opacity, torch.sigmoid(m._opacity))
