@@ -57,14 +57,15 @@ struct GeomLayout {
 };
 
 struct BinLayout {
-  size_t keys_a, keys_b, vals_a, vals_b, scratch, total;
+  size_t keys_a, keys_b, vals_a, vals_b, scratch, chunk, total;
   explicit BinLayout(size_t K) {
     size_t o = 0;
     keys_a = o; o += align_up(K * 4);
     keys_b = o; o += align_up(K * 4);
     vals_a = o; o += align_up(K * 4);
     vals_b = o; o += align_up(K * 4);
-    scratch = o; o += align_up(std::max(dmgs::radix_scratch_words(K), dmgs::duplicate_scratch_words(K)) * 4);
+    scratch = o; o += align_up(dmgs::radix_scratch_words(K) * 4);  // the duplicate may fill the first histogram
+    chunk = o; o += align_up(dmgs::duplicate_scratch_words(K) * 4);
     total = o;
   }
 };
@@ -263,11 +264,27 @@ int dmgs_forward_binning(const dmgs_settings* s, int64_t P, int64_t K, const voi
     d.tile_keys = ka;
     d.values = va;
     d.K = (uint32_t)K;
-    d.chunk_first = at<uint32_t>(binning_ws, BL.scratch);
+    d.chunk_first = at<uint32_t>(binning_ws, BL.chunk);
+    // two tile-sort passes (9..16 tile bits) on 4096-key blocks: the duplicate
+    // counts the first pass's digits per output chunk, so that pass needs no
+    // histogram launch
+    const int bits = tile_bits(T);
+    const int w0 = (bits > 8 && bits <= 16) ? dmgs::radix_min_passes_first_width((size_t)K, bits) : -1;
+    uint32_t* scratch = at<uint32_t>(binning_ws, BL.scratch);
+    if (w0 > 0) {
+      d.hist = scratch;
+      d.hist_bits = w0;
+    }
     dmgs::launch_duplicate(d, st);
     if ((rc = check_launch("duplicate", st, s->debug))) return rc;
-    dmgs::radix_sort_pairs(ka, va, at<uint32_t>(binning_ws, BL.keys_b), at<uint32_t>(binning_ws, BL.vals_b),
-                           (size_t)K, tile_bits(T), at<uint32_t>(binning_ws, BL.scratch), st);
+    uint32_t* kb = at<uint32_t>(binning_ws, BL.keys_b);
+    uint32_t* vb = at<uint32_t>(binning_ws, BL.vals_b);
+    if (w0 > 0) {
+      if (dmgs::radix_sort_pairs_min_passes(ka, va, kb, vb, (size_t)K, bits, scratch, st, nullptr, true))
+        return fail(DMGS_EHIP, "internal: tile sort ended in the second buffer");
+    } else {
+      dmgs::radix_sort_pairs(ka, va, kb, vb, (size_t)K, bits, scratch, st);
+    }
     if ((rc = check_launch("tile sort", st, s->debug))) return rc;
     dmgs::launch_tile_ranges(ka, (size_t)K, ranges, T, st);
   } else {
