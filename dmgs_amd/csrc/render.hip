@@ -749,9 +749,17 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
     }
     bg_term[k] = -T_final * (c.bg0 * g0[k] + c.bg1 * g1[k] + c.bg2 * g2[k]);
     st[k] = bwd_pixel(T_final);
-    m = max(m, last_contributor[k]);
   }
-  for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+  // per strip, the largest last contributor of its 64 pixels (wave-uniform):
+  // list positions at or past it cannot contribute to the strip
+  uint32_t strip_max[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    uint32_t x = last_contributor[k];
+    for (int off = 32; off > 0; off >>= 1) x = max(x, (uint32_t)__shfl_xor((int)x, off, 64));
+    strip_max[k] = __builtin_amdgcn_readfirstlane(x);
+    m = max(m, strip_max[k]);
+  }
   const int todo_total = min((int)(c.range.y - c.range.x), (int)m);
 
   const unsigned row_lane = lane & 15;
@@ -797,10 +805,15 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       if (reach == 0u) continue;
       uint32_t contributor;
       if constexpr (CELLS) {
-        contributor = splat_pos(L, j);
+        contributor = __builtin_amdgcn_readfirstlane(splat_pos(L, j));
       } else {
         contributor = (uint32_t)(top - 1 - j);
       }
+      uint32_t live = reach;  // strips the splat reaches and that still have pixels behind it
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (contributor >= strip_max[k]) live &= ~(1u << k);
+      if (live == 0u) continue;
       const float2 xy = make_float2(L.xym[j].x, L.xym[j].y);
       const float4 q = L.qo[j];
       const float4 col = L.rgb_id[j];
@@ -812,7 +825,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       bool any = false;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        if (!((reach >> k) & 1u)) continue;  // wave-uniform
+        if (!((live >> k) & 1u)) continue;  // wave-uniform
         const float dy = xy.y - pyf[k];
         const float p = power_at(pd, q.z, dy);
         const float G = splat_exp2(p);
