@@ -299,6 +299,7 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
   const dmgs::MVArgs a = mv_args(b, c, params);
   uint32_t* offs = at<uint32_t>(b->state, c.S.offsets);
   uint32_t* status = at<uint32_t>(b->state, c.S.status);
+  bool chunks_ready = false;  // the duplicate's chunk table was filled by this call's scan
   if (stages & DMGS_STAGE_GEOMETRY) {
     dmgs::MVArgs ap = a;
     ap.status = status;  // zeroed by the preprocess, set by the compaction
@@ -313,7 +314,14 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     dmgs::radix_sort_depth_keys(keys, a.order, at<uint32_t>(b->state, c.S.keys_b),
                                 at<uint32_t>(b->state, c.S.order_b), N, gs, st, V_dev);
     dmgs::launch_gather_rects(a.rects, a.order, at<uint2>(b->state, c.S.rects_sorted), offs, N, st, V_dev);
-    dmgs::launch_inclusive_scan(offs, offs, N, gs, st);
+    if ((stages & DMGS_STAGE_BINNING) && b->capacity > 0 && b->lists) {
+      // the binning follows in this call: the scan also fills the duplicate's chunk table
+      dmgs::launch_inclusive_scan_chunks(offs, offs, N, gs, at<uint32_t>(b->lists, c.L.chunk),
+                                         (uint32_t)b->capacity, st);
+      chunks_ready = true;
+    } else {
+      dmgs::launch_inclusive_scan(offs, offs, N, gs, st);
+    }
     if ((rc = bcheck("batch preprocess / depth sort / scan", st))) return rc;
   }
   if (stages & DMGS_STAGE_BINNING) {
@@ -331,6 +339,7 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     d.K = K;
     d.K_dev = c.count(b);
     d.chunk_first = at<uint32_t>(b->lists, c.L.chunk);
+    d.chunk_ready = chunks_ready;
     d.view_major = b->cell_mode == 1;
     // the duplicate also zeroes the tile-order histogram and the cell ranges (adjacent)
     d.zero = at<uint32_t>(b->state, c.S.bucket_hist);

@@ -39,6 +39,10 @@ void launch_mark_visible(int P, const float* means3D, const float* viewmatrix, u
 // Inclusive scan of n uint32 values; scratch must hold scan_scratch_words(n) words.
 size_t scan_scratch_words(size_t n);
 void launch_inclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch, hipStream_t s);
+// The same over instance counts, also filling the duplicate's chunk table
+// (DupArgs::chunk_first for K output slots; then set DupArgs::chunk_ready).
+void launch_inclusive_scan_chunks(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch,
+                                  uint32_t* chunk_first, uint32_t K, hipStream_t s);
 
 // Stable LSD radix sort of (key, value) pairs over bits [0, bits), in passes of
 // at most 8 bits.  Data starts in (k0, v0); (k1, v1) is the ping-pong buffer.
@@ -83,6 +87,7 @@ struct DupArgs {
   uint32_t K;                   // capacity of tile_keys / values
   const uint32_t* K_dev;        // optional device count (clamped to K)
   uint32_t* chunk_first;        // duplicate_scratch_words(K) words
+  bool chunk_ready;             // chunk_first already filled (launch_inclusive_scan_chunks)
   bool view_major;
   // optional: the first radix pass's histogram (digits = the low hist_bits of
   // a key), written per 4096-slot chunk into the sort scratch

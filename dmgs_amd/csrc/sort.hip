@@ -81,10 +81,15 @@ __device__ __forceinline__ int scan_pad(int p) { return p + (p >> 5); }
 // The tile moves between HBM and registers through LDS: global loads and
 // stores stay fully coalesced (consecutive lanes, consecutive words) while
 // each thread scans 16 consecutive elements.
+// chunk_first (inclusive scans of instance counts): also record, for every
+// 4096-slot output chunk below K, the element whose [start, end) holds the
+// chunk's first slot (what chunk_first_kernel computes from the offsets).
+constexpr uint32_t kChunkSlots = 4096;
 template <bool kExclusive>
 __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* __restrict__ in,
                                                                  uint32_t* __restrict__ out, size_t n,
-                                                                 const uint32_t* __restrict__ spine) {
+                                                                 const uint32_t* __restrict__ spine,
+                                                                 uint32_t* __restrict__ chunk_first, uint32_t K) {
   __shared__ uint32_t s_wave[kScanThreads / 64];
   __shared__ uint32_t tile[kScanTile + kScanTile / 32];
   const size_t base = (size_t)blockIdx.x * kScanTile;
@@ -107,6 +112,12 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
     const uint32_t x = v[i];
+    if (!kExclusive && chunk_first && x > 0u) {
+      const uint32_t beg = min(run, K), end = min(run + x, K);
+      const size_t idx = base + (size_t)threadIdx.x * kScanItems + i;
+      for (uint32_t c = (beg + kChunkSlots - 1) / kChunkSlots; c * kChunkSlots < end; ++c)
+        chunk_first[c] = (uint32_t)idx;
+    }
     if (!kExclusive) run += x;
     tile[scan_pad((int)threadIdx.x * kScanItems + i)] = run;
     if (kExclusive) run += x;
@@ -123,17 +134,22 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
 size_t scan_scratch_words(size_t n) { return (n + kScanTile - 1) / kScanTile + 64; }
 
 static void scan_impl(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch, bool exclusive,
-                      hipStream_t s) {
+                      hipStream_t s, uint32_t* chunk_first = nullptr, uint32_t K = 0) {
   if (n == 0) return;
   const size_t nb = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, scratch);
   hipLaunchKernelGGL(scan_spine_kernel, dim3(1), dim3(kScanThreads), 0, s, scratch, nb);
   if (exclusive)
     hipLaunchKernelGGL(scan_apply_kernel<true>, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n,
-                       (const uint32_t*)scratch);
+                       (const uint32_t*)scratch, (uint32_t*)nullptr, 0u);
   else
     hipLaunchKernelGGL(scan_apply_kernel<false>, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n,
-                       (const uint32_t*)scratch);
+                       (const uint32_t*)scratch, chunk_first, K);
+}
+
+void launch_inclusive_scan_chunks(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch,
+                                  uint32_t* chunk_first, uint32_t K, hipStream_t s) {
+  scan_impl(in, out, n, scratch, false, s, chunk_first, K);
 }
 
 void launch_inclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch, hipStream_t s) {
@@ -477,6 +493,7 @@ void launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t* dst, 
 // slots back to its Gaussian by binary search, so stores are coalesced and a
 // few huge splats cannot serialise a workgroup.
 constexpr uint32_t kDupChunk = 4096;
+static_assert(kDupChunk == kChunkSlots, "the scan fills the duplicate's chunk table");
 
 __device__ __forceinline__ uint32_t device_count(uint32_t cap, const uint32_t* dev) {
   return dev ? min(cap, *dev) : cap;
@@ -652,7 +669,7 @@ size_t duplicate_scratch_words(size_t K) { return (K + kDupChunk - 1) / kDupChun
 
 void launch_duplicate(const DupArgs& a, hipStream_t s) {
   if (a.P <= 0 || a.K == 0) return;
-  hipLaunchKernelGGL(chunk_first_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  if (!a.chunk_ready) hipLaunchKernelGGL(chunk_first_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
   const uint32_t chunks = (a.K + kDupChunk - 1) / kDupChunk;
   hipLaunchKernelGGL(duplicate_kernel, dim3(chunks), dim3(256), 0, s, a);
 }
