@@ -27,6 +27,7 @@ same workload; ``cpu_baseline`` times the FP64 CPU oracle on a bounded
 sample of one 800x800 view.
 """
 import argparse
+import gc
 import json
 import math
 import os
@@ -243,6 +244,8 @@ def serving_run(n_requests, rate, sweep_rates=(100.0, 400.0, 1600.0)):
     SV.serve_once(srv, SV.poisson_requests(names, 16, None, seed=99))  # warm-up
     # warm-up at a moderate rate: the first batches of each (scene, size class) time both blend shapes
     SV.serve_once(srv, SV.poisson_requests(names, 150, 400.0, seed=98))
+    gc.collect()
+    gc.freeze()
     stats = SV.serve_once(srv, SV.poisson_requests(names, n_requests, rate, seed=3))
     stats.update(rate_per_s=rate, requests=n_requests, budget_GB=round(srv.budget / 1e9, 1),
                  peak_reserved_GB=round(srv.peak_reserved / 1e9, 3), max_in_flight=srv.max_active)
@@ -434,6 +437,10 @@ def main():
         ab["geometry_backward"]
 
     # ---- forward-only latency: batch, and the plain single view ----
+    # the objects made so far move to the permanent GC generation, so a full
+    # collection of them cannot land inside a timed render (as in a server)
+    gc.collect()
+    gc.freeze()
     blat = []
     for _ in range(max(5, args.render_samples // 5)):
         torch.cuda.synchronize()
