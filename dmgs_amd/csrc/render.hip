@@ -180,8 +180,11 @@ __device__ __forceinline__ BwdPixel bwd_pixel(float T_final) { return {T_final, 
 // dL/dalpha; w = alpha T is dL/dcolour_ch / g_ch.
 __device__ __forceinline__ float bwd_step(BwdPixel& st, float alpha, const float4& col, float g0, float g1,
                                           float g2, float bg_term, float& w) {
-  // each state value is read before it is replaced, so it can be updated in place
-  st.S = fmaf(st.om, st.S, st.pcg);
+  // each state value is read before it is replaced, so it can be updated in
+  // place; the FMA names its destination (S's own register) because the
+  // compiler's two-address form would land it in pcg's register and cost two
+  // moves per pixel to restore the loop's registers
+  asm("v_fma_f32 %0, %1, %0, %2" : "+v"(st.S) : "v"(st.om), "v"(st.pcg));
   st.om = 1.f - alpha;
   const float inv = __builtin_amdgcn_rcpf(st.om);  // T before this splat = T / (1 - alpha)
   st.T *= inv;
