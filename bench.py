@@ -338,6 +338,29 @@ def _cpu_cam(cam):
     return c
 
 
+def _read_valu(kernel):
+    """Wave64 VALU instructions per launch of ``kernel`` at the default
+    workload (rocprofv3 SQ_INSTS_VALU, committed under profiles/), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "valu.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def _valu_issue(kernel, ms, args):
+    """The kernel's VALU issue rate against MI355X's (256 CUs x 4 SIMDs, one
+    wave64 instruction per SIMD per 4 cycles at 2.4 GHz), from its committed
+    instruction count and the live time; only for the default workload."""
+    n = _read_valu(kernel)
+    if n is None or args.workload != "views" or args.gaussians or args.views or (args.width, args.height) != (800, 800):
+        return None
+    peak = 256 * 4 * 2.4e9 / 4  # wave instructions per second
+    return {"instructions": n, "achieved_per_s": round(n / (ms * 1e-3), 1), "peak_per_s": peak,
+            "frac": round(n / (ms * 1e-3) / peak, 4)}
+
+
 def _read_traffic(kernel):
     """HBM bytes per launch of ``kernel`` measured with rocprofv3 PMC counters
     (committed under profiles/), or None when no measurement exists."""
@@ -493,7 +516,8 @@ def main():
                      "kernel": kern, "kernel_ms": round(stage_ms[kern], 4), "algorithmic_bytes": ab[kern],
                      "note": "algorithmic bytes = 8T + 40K + 20X (+36V backward) with K the cell instances; "
                              "measured traffic (rocprofv3 FETCH/WRITE, copy-calibrated) is far lower: the blend "
-                             "kernels are VALU-bound"},
+                             "kernels are VALU-bound",
+                     "valu_issue": _valu_issue(f"batched_{kern}", stage_ms[kern], args)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "autotune_ms": {str(k): round(v, 4) for k, v in tune.items()},
         "render_ms_p50": round(float(np.percentile(plat, 50)), 3),
