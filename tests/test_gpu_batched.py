@@ -93,7 +93,10 @@ def _batched(model_cpu, cams, bgs, variant=0, cell_mode=None):
 
 @pytest.mark.parametrize("n,sizes", [(20000, [(128, 96), (200, 200), (160, 90)]),
                                      (5000, [(64, 48)]),
-                                     (30000, [(256, 256), (96, 160), (33, 17), (200, 120)])])
+                                     (30000, [(256, 256), (96, 160), (33, 17), (200, 120)]),
+                                     # more than 16 views: global cell lists and the per-view
+                                     # compaction loop
+                                     (8000, [(48, 40), (40, 48), (64, 32)] * 6 + [(24, 24), (56, 40)])])
 def test_batched_matches_single_view(n, sizes):
     print(f"case: batched vs single-view n={n} sizes={sizes}", flush=True)
     model = S.free_scene(n, seed=0, layout="mixture", shell_radius=3.0)
