@@ -100,7 +100,10 @@ def _batched(model_cpu, cams, bgs, variant=0, cell_mode=None):
 def test_batched_matches_single_view(n, sizes):
     print(f"case: batched vs single-view n={n} sizes={sizes}", flush=True)
     model = S.free_scene(n, seed=0, layout="mixture", shell_radius=3.0)
-    cams = _cameras(sizes)
+    _check_against_single_view(model, _cameras(sizes))
+
+
+def _check_against_single_view(model, cams):
     bgs = _bgs(len(cams))
     r, params = _batched(model, cams, bgs)
     dL = _dL(r)
@@ -121,6 +124,36 @@ def test_batched_matches_single_view(n, sizes):
     for k in RAW:
         e = grad_rel_err(bg[k], pgrads[k])
         assert e < GRAD_TOL, f"{k}: rel err {e}"
+    return r, images, bg
+
+
+def _away_camera(w, h, device="cuda"):
+    """A camera outside the scene's shell that looks away from it: nothing is visible."""
+    p = np.array([0.0, 0.0, 8.0])
+    R = C.look_at(p, target=tuple(2.0 * p))
+    f = math.radians(50.0)
+    return C.SimpleCamera(R, -R.T @ p, f, f, w, h, device=device)
+
+
+@pytest.mark.parametrize("kinds", [("scene", "away", "scene", "scene", "scene"), ("away", "away")])
+def test_batched_empty_and_tiny_views(kinds):
+    """Views that see nothing and 1-pixel-high views in one batch; a batch in
+    which no view sees anything renders the backgrounds and has zero gradients."""
+    sizes = [(96, 64), (64, 48), (1, 1), (17, 1), (1, 23)]
+    print(f"case: empty and tiny views {kinds}", flush=True)
+    model = S.free_scene(4000, seed=6, layout="mixture", shell_radius=3.0)
+    pos = C.sphere_positions(len(kinds), 4.0, 9)
+    cams = [C.camera_at(p, w, h, fov_deg=50.0) if k == "scene" else _away_camera(w, h)
+            for k, p, (w, h) in zip(kinds, pos, sizes)]
+    r, images, grads = _check_against_single_view(model, cams)
+    bgs = _bgs(len(cams))
+    for v, k in enumerate(kinds):
+        if k == "away":
+            img = r.image(images, v)
+            assert torch.equal(img, bgs[v].to(img.device)[:, None, None].expand_as(img))
+    if "scene" not in kinds:
+        for name in RAW:
+            assert grads[name].abs().max().item() == 0.0, name
 
 
 def test_batched_against_oracle():
