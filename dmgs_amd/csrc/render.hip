@@ -720,7 +720,6 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   // counter is in order) and they complete while that batch is computed.
   __shared__ float s_acc[2][BATCH][DMGS_GACC_USED];
   __shared__ uint32_t s_gid[2][BATCH];
-  __shared__ float4 s_q[2][BATCH];  // scaled conic and opacity: the flush applies them to the reduced sums
   __shared__ uint32_t s_wc[1];
   const TileCtx c = tile_ctx<16>(a);  // pixel k = 0 of this lane: (lx, ly), ly in 0..3
   const unsigned lane = threadIdx.x & 63;
@@ -771,13 +770,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   auto flush = [&](int buf, int cnt) {
     for (int i = threadIdx.x; i < cnt * DMGS_GACC_USED; i += NT) {
       const int slot = i / DMGS_GACC_USED, q = i - slot * DMGS_GACC_USED;
-      const float* sa = s_acc[buf][slot];
-      float raw = sa[q];
-      if (q < 2 || q == 5) {  // sums of t dx, t dy, t: the conic and 1/o enter once per splat here
-        const float4 qo = s_q[buf][slot];
-        raw = q == 0 ? dmean_x(qo, sa[0], sa[1]) : (q == 1 ? dmean_y(qo, sa[0], sa[1]) : sa[5] / qo.w);
-      }
-      const float val = raw * grad_scale(q, ddelx_dx, ddely_dy);
+      const float val = s_acc[buf][slot][q] * grad_scale(q, ddelx_dx, ddely_dy);
       if (val != 0.f) unsafeAtomicAdd(a.gacc + (size_t)s_gid[buf][slot] * DMGS_GACC_STRIDE + q, val);
     }
   };
@@ -802,10 +795,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       n = min(BATCH, top);
       cursor -= n;
     }
-    for (int s = threadIdx.x; s < n; s += NT) {
-      s_gid[buf][s] = __float_as_uint(L.rgb_id[s].w);
-      s_q[buf][s] = L.qo[s];
-    }
+    for (int s = threadIdx.x; s < n; s += NT) s_gid[buf][s] = __float_as_uint(L.rgb_id[s].w);
     // Every staging load has landed here; saying so keeps the compiler from
     // waiting later (behind the atomics below) for a load it cannot rule out.
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
@@ -859,7 +849,8 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       }
       if (__ballot(any) == 0) continue;
       const float s_tdx = s_t * dx;
-      const float v[DMGS_GACC_USED] = {s_tdx, s_tdy, s_tdx * dx, s_tdy * dx, s_tdy2, s_t, c0, c1, c2};
+      const float v[DMGS_GACC_USED] = {dmean_x(q, s_tdx, s_tdy), dmean_y(q, s_tdx, s_tdy), s_tdx * dx, s_tdy * dx,
+                                       s_tdy2, s_t * __builtin_amdgcn_rcpf(q.w), c0, c1, c2};
       const float mine = row_sums9(v, row_lane);
       if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[buf][j][row_lane], mine);
     }
