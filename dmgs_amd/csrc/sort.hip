@@ -441,7 +441,10 @@ static void radix_dispatch(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v
 void radix_sort_depth_keys(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, uint32_t* scratch,
                            hipStream_t s, const uint32_t* n_dev) {
   if (n == 0) return;
-  radix_dispatch(k0, v0, k1, v1, n, 32, scratch, s, n_dev, 4, false, 24);
+  // every pass counts with per-wave LDS atomics: even the top byte (view and
+  // exponent bits) spreads over enough digits that they rarely collide
+  // (measured 5 us per sort faster than ballot grouping on 5.9M keys)
+  radix_dispatch(k0, v0, k1, v1, n, 32, scratch, s, n_dev, 4, false, 32);
 }
 
 static int min_passes(int bits) { return bits <= 8 ? 1 : (bits <= 16 ? 2 : 4); }
