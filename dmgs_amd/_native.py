@@ -97,6 +97,7 @@ _SIGNATURES = {
                                              ctypes.POINTER(c_int64)]),
     "dmgs_batched_list_offsets": (c_int32, [ctypes.POINTER(DmgsBatch), ctypes.POINTER(c_size_t),
                                             ctypes.POINTER(c_size_t)]),
+    "dmgs_batched_cell_sort_bits": (c_int32, [ctypes.POINTER(DmgsBatch)]),
     "dmgs_batched_tile_view": (c_int32, [ctypes.POINTER(DmgsView), c_int32, c_void_p, c_int32]),
     "dmgs_batched_state_size": (c_size_t, [c_int32, c_int64, c_int32, c_int64]),
     "dmgs_batched_lists_size": (c_size_t, [c_int64]),

@@ -143,10 +143,16 @@ struct Ctx {
         S(N, (size_t)b->total_tiles, (size_t)b->total_pixels,
           dmgs::mv_wave_count_words((int)b->num_gaussians, b->num_views)),
         L((size_t)b->capacity), mode(b->cell_mode),
-        local_bits(b->cell_mode == 1 ? std::max(1, bits_for((uint32_t)(b->max_view_cells - 1))) : 0) {}
+        local_bits(b->cell_mode == 1 ? std::max(1, bits_for((uint32_t)(b->max_view_cells - 1))) : 0),
+        global_bits(std::max(1, bits_for((uint32_t)(b->total_cells - 1)))) {}
   uint32_t* count(const dmgs_batch* b) const { return at<uint32_t>(b->state, S.offsets) + (N - 1); }
   int mode;
+  // key bits of the cell sort: the cell within the view (mode 1) or the
+  // global cell id (mode 0; cell ids 0 .. total_cells - 1, so 65536 cells
+  // need 16 bits and bit 16, the first tile-mask bit, stays out of the sort)
   int local_bits;
+  int global_bits;
+  int cell_sort_bits() const { return mode == 1 ? local_bits : global_bits; }
   // a one-pass sort (view-major lists, at most 256 cells per view) leaves the lists in the b buffers
   bool lists_in_b() const { return mode == 1 && local_bits <= 8; }
 };
@@ -284,6 +290,12 @@ int dmgs_batched_list_offsets(const dmgs_batch* b, size_t* keys, size_t* ids) {
   return DMGS_OK;
 }
 
+int dmgs_batched_cell_sort_bits(const dmgs_batch* b) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  return Ctx(b).cell_sort_bits();
+}
+
 int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images, int32_t stages,
                          dmgs_stream_t stream) {
   int rc = check_batch(b);
@@ -362,7 +374,7 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
       dmgs::launch_view_cell_ranges(in_b ? kb : d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells,
                                     a.views, st, d.K_dev, true);
     } else {
-      dmgs::radix_sort_pairs(d.tile_keys, d.values, kb, vb, K, bits_for((uint32_t)b->total_cells),
+      dmgs::radix_sort_pairs(d.tile_keys, d.values, kb, vb, K, c.global_bits,
                              at<uint32_t>(b->lists, c.L.scratch), st, d.K_dev);
       dmgs::launch_tile_ranges(d.tile_keys, K, at<uint2>(b->state, c.S.ranges), b->total_cells, st, d.K_dev,
                                DMGS_CELL_KEY_MASK);

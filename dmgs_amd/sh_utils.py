@@ -1,10 +1,35 @@
+#  Copyright 2021 The PlenOctree Authors.
+#  Redistribution and use in source and binary forms, with or without
+#  modification, are permitted provided that the following conditions are met:
+#
+#  1. Redistributions of source code must retain the above copyright notice,
+#  this list of conditions and the following disclaimer.
+#
+#  2. Redistributions in binary form must reproduce the above copyright notice,
+#  this list of conditions and the following disclaimer in the documentation
+#  and/or other materials provided with the distribution.
+#
+#  THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS AND CONTRIBUTORS "AS IS"
+#  AND ANY EXPRESS OR IMPLIED WARRANTIES, INCLUDING, BUT NOT LIMITED TO, THE
+#  IMPLIED WARRANTIES OF MERCHANTABILITY AND FITNESS FOR A PARTICULAR PURPOSE
+#  ARE DISCLAIMED. IN NO EVENT SHALL THE COPYRIGHT HOLDER OR CONTRIBUTORS BE
+#  LIABLE FOR ANY DIRECT, INDIRECT, INCIDENTAL, SPECIAL, EXEMPLARY, OR
+#  CONSEQUENTIAL DAMAGES (INCLUDING, BUT NOT LIMITED TO, PROCUREMENT OF
+#  SUBSTITUTE GOODS OR SERVICES; LOSS OF USE, DATA, OR PROFITS; OR BUSINESS
+#  INTERRUPTION) HOWEVER CAUSED AND ON ANY THEORY OF LIABILITY, WHETHER IN
+#  CONTRACT, STRICT LIABILITY, OR TORT (INCLUDING NEGLIGENCE OR OTHERWISE)
+#  ARISING IN ANY WAY OUT OF THE USE OF THIS SOFTWARE, EVEN IF ADVISED OF THE
+#  POSSIBILITY OF SUCH DAMAGE.
 """Real spherical harmonics for the ``convert_SHs_python`` path.
 
-Same polynomials and constants as utils/sh_utils.py:26-110 (eval_sh, degrees
-0-4) and RGB2SH / SH2RGB (utils/sh_utils.py:118-124).  This runs as plain torch
-ops on the device because that is what the reference's adapter does when
-``pipe.convert_SHs_python`` is set (gaussian_renderer/__init__.py:72-78); the
-rasterizer's own SH path is the fused HIP preprocess kernel.
+The constants and polynomials (degrees 0-3) and RGB2SH / SH2RGB are those of
+utils/sh_utils.py:26-124, which carries the PlenOctree notice above; they are
+kept term for term so the FP32 colours equal the reference adapter's.  This
+runs as plain torch ops on the device because that is what the reference's
+adapter does when ``pipe.convert_SHs_python`` is set
+(gaussian_renderer/__init__.py:72-78); the rasterizer's own SH path is the
+fused HIP preprocess kernel.  Degree 4 is not provided: the rasterizer and the
+models here stop at degree 3 (16 coefficients per colour).
 """
 import torch
 
@@ -13,13 +38,12 @@ C1 = 0.4886025119029199
 C2 = [1.0925484305920792, -1.0925484305920792, 0.31539156525252005, -1.0925484305920792, 0.5462742152960396]
 C3 = [-0.5900435899266435, 2.890611442640554, -0.4570457994644658, 0.3731763325901154, -0.4570457994644658,
       1.445305721320277, -0.5900435899266435]
-C4 = [2.5033429417967046, -1.7701307697799304, 0.9461746957575601, -0.6690465435572892, 0.10578554691520431,
-      -0.6690465435572892, 0.47308734787878004, -1.7701307697799304, 0.6258357354491761]
 
 
 def eval_sh(deg: int, sh: torch.Tensor, dirs: torch.Tensor) -> torch.Tensor:
-    """sh (..., C, (deg+1)^2 or more), dirs (..., 3) unit -> (..., C)."""
-    assert 0 <= deg <= 4 and (deg + 1) ** 2 <= sh.shape[-1]
+    """sh (..., C, (deg+1)^2 or more), dirs (..., 3) unit -> (..., C); deg in [0, 3]."""
+    if not 0 <= deg <= 3 or (deg + 1) ** 2 > sh.shape[-1]:
+        raise ValueError(f"eval_sh: degree {deg} needs 0 <= deg <= 3 and {(deg + 1) ** 2} coefficients")
     result = C0 * sh[..., 0]
     if deg > 0:
         x, y, z = dirs[..., 0:1], dirs[..., 1:2], dirs[..., 2:3]
@@ -36,15 +60,6 @@ def eval_sh(deg: int, sh: torch.Tensor, dirs: torch.Tensor) -> torch.Tensor:
                           + C3[3] * z * (2 * zz - 3 * xx - 3 * yy) * sh[..., 12]
                           + C3[4] * x * (4 * zz - xx - yy) * sh[..., 13] + C3[5] * z * (xx - yy) * sh[..., 14]
                           + C3[6] * x * (xx - 3 * yy) * sh[..., 15])
-                if deg > 3:
-                    result = (result + C4[0] * xy * (xx - yy) * sh[..., 16]
-                              + C4[1] * yz * (3 * xx - yy) * sh[..., 17] + C4[2] * xy * (7 * zz - 1) * sh[..., 18]
-                              + C4[3] * yz * (7 * zz - 3) * sh[..., 19]
-                              + C4[4] * (zz * (35 * zz - 30) + 3) * sh[..., 20]
-                              + C4[5] * xz * (7 * zz - 3) * sh[..., 21]
-                              + C4[6] * (xx - yy) * (7 * zz - 1) * sh[..., 22]
-                              + C4[7] * xz * (xx - 3 * yy) * sh[..., 23]
-                              + C4[8] * (xx * (xx - 3 * yy) - yy * (3 * xx - yy)) * sh[..., 24])
     return result
 
 

@@ -332,6 +332,11 @@ typedef struct dmgs_batch {
  * of this batch leaves (they depend on cell_mode and max_view_cells). */
 int dmgs_batched_list_offsets(const dmgs_batch* batch, size_t* keys, size_t* ids);
 
+/* Number of low key bits the cell-list sort of this batch covers (the cell
+ * id in cell_mode 0, the cell within the view in mode 1; never the tile-mask
+ * bits above bit 15), or a negative error code.  Host-side query. */
+int dmgs_batched_cell_sort_bits(const dmgs_batch* batch);
+
 /* Stage mask of the batched calls.  Forward: GEOMETRY = preprocess, depth
  * sort and the scan of tile counts (afterwards the instance count is at
  * offsets[1]); BINNING = duplicate, tile sort, ranges; BLEND = the blend.

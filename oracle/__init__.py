@@ -23,7 +23,6 @@ restated from the published upstream algorithm and are marked [U]):
                  (scene/gaussian_geo_model_finetune.py:396-482) and stage 2
                  (scene/gaussian_geo_model_mlp_flex.py:258-385), FP64 autograd.
 * ``loss``     – L1 and SSIM (utils/loss_utils.py:17-62), FP64.
-* ``scenes``   – seeded synthetic scenes for the benchmark configurations.
 
 Pinning status (what fixes each function independently of its own formula)
 is listed in each module header.  Functions with no independent pin say

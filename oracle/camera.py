@@ -7,7 +7,7 @@
                         full_proj = world_view @ P^T, camera_center =
                         inverse(world_view)[3, :3]; znear 0.01, zfar 100.
 
-Pins (tests/test_oracle_camera.py): frustum corners map to NDC +-1, FoV 90 deg
+Pins (tests/test_oracle_sh_camera.py): frustum corners map to NDC +-1, FoV 90 deg
 gives focal = W/2, the camera centre maps to the view-space origin.
 """
 import math

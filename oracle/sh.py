@@ -4,7 +4,7 @@ Restates utils/sh_utils.py:26-110 (eval_sh, constants :41-58; the CUDA
 rasterizer [U] uses the same constants and polynomials, then adds 0.5 and
 clamps at 0).
 
-Pin (tests/test_oracle_sh.py): every basis function is compared against
+Pin (tests/test_oracle_sh_camera.py): every basis function is compared against
 ``scipy.special.sph_harm`` turned into real harmonics with the Condon-Shortley
 phase removed, which fixes each constant, sign and polynomial independently of
 this file; degree-0 and axis directions are checked as special cases.

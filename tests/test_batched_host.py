@@ -106,6 +106,24 @@ def test_batch_descriptor_layout():
     assert ctypes.sizeof(N.DmgsView) == 208
 
 
+def test_cell_sort_width():
+    """The cell-list sort covers exactly the bits of the largest cell id:
+    global cell ids 0 .. total_cells - 1 in cell mode 0 (65536 cells need 16
+    bits, so bit 16 — the first bit of the tile mask above the cell id —
+    never enters the sort and a cell's entries stay one run), the cell within
+    the view in mode 1."""
+    lib = _lib()
+    for cells, bits in ((1, 1), (2, 1), (3, 2), (256, 8), (257, 9), (32768, 15), (32769, 16), (65535, 16),
+                        (65536, 16)):
+        got = lib.dmgs_batched_cell_sort_bits(_desc(total_cells=cells, total_tiles=16 * cells, num_views=20))
+        assert got == bits, (cells, got)
+    for vc, bits in ((1, 1), (169, 8), (256, 8), (257, 9), (4096, 12)):
+        got = lib.dmgs_batched_cell_sort_bits(_desc(cell_mode=1, max_view_cells=vc, total_cells=2 * vc,
+                                                    total_tiles=32 * vc))
+        assert got == bits, (vc, got)
+    assert lib.dmgs_batched_cell_sort_bits(_desc(total_cells=65537, total_tiles=16 * 65537)) == -1
+
+
 def test_entry_points_validate_before_launch():
     lib = _lib()
     fake = ctypes.c_void_p(0x1000)

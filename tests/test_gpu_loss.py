@@ -1,5 +1,5 @@
-"""GPU parity of the fused L1 + SSIM loss against the FP64 oracle and the
-reference-form torch implementation (utils/loss_utils.py:17-62)."""
+"""GPU parity of the fused L1 + SSIM loss against the FP64 oracle
+(utils/loss_utils.py:17-62, train_geo_stage3.py:122-124)."""
 import pytest
 import torch
 
@@ -25,12 +25,6 @@ def test_fused_dssim_l1_matches_oracle(C, H, W):
     e = grad_rel_err(xg.grad, ref_grad)
     print(f"  value diff {abs(val.item() - ref_val):.2e}, grad rel err {e:.2e}")
     assert e < 1e-3
-    # the torch reference form on the device agrees too
-    xt = x.cuda().requires_grad_(True)
-    tv = L.dssim_l1(xt, y.cuda(), 0.2)
-    tv.backward()
-    assert abs(tv.item() - val.item()) < 1e-5
-    assert grad_rel_err(xg.grad, xt.grad) < 1e-3
 
 
 def test_fused_dssim_l1_identical_images():

@@ -181,6 +181,70 @@ def test_cov3d_invariants():
     assert torch.allclose(OR.cov3d_from_scale_rot(s, -q, 1.7), c, atol=1e-12)
 
 
+def _anisotropic_preprocess(n=400, seed=3):
+    """Strongly anisotropic Gaussians in front of the camera (random
+    rotations, scales spread over two decades)."""
+    st = _settings(160, 120, fov=55.0)
+    g = torch.Generator().manual_seed(seed)
+    means = (torch.rand(n, 3, generator=g, dtype=F64) * 2 - 1) * torch.tensor([0.8, 0.6, 1.5], dtype=F64)
+    scales = torch.exp(torch.rand(n, 3, generator=g, dtype=F64) * 4.6 - 5.5)
+    rots = torch.randn(n, 4, generator=g, dtype=F64)
+    pre = OR.preprocess(means, torch.full((n, 1), 0.5, dtype=F64), st, colors_precomp=torch.ones(n, 3),
+                        scales=scales, rotations=rots)
+    return st, pre
+
+
+def test_conic_is_the_inverse_of_cov2d():
+    """conic @ cov2D = I for every Gaussian (2x2 matrix product, independent
+    of the closed-form adjugate the oracle uses)."""
+    _, pre = _anisotropic_preprocess()
+    vis = pre["visible"]
+    assert vis.sum() > 100
+    a, b, c = pre["cov2D"][vis].unbind(1)
+    A = torch.stack([a, b, b, c], 1).reshape(-1, 2, 2)
+    ca, cb, cc = pre["conic"][vis].unbind(1)
+    Q = torch.stack([ca, cb, cb, cc], 1).reshape(-1, 2, 2)
+    eye = torch.eye(2, dtype=F64).expand_as(A)
+    assert torch.allclose(Q @ A, eye, atol=1e-9)
+    assert torch.allclose(A @ Q, eye, atol=1e-9)
+    # the conic is symmetric positive definite (a > 0, det > 0)
+    assert (ca > 0).all() and (ca * cc - cb * cb > 0).all()
+
+
+def test_radius_is_three_sigma_of_the_largest_eigenvalue():
+    """radius = ceil(3 sqrt(lambda_max(cov2D))) from a symmetric eigen
+    solver wherever the eigenvalue gap is wide ((a - c)^2 / 4 + b^2 >= 0.1);
+    where it is narrower the reference floors the root at sqrt(0.1), so
+    radius = ceil(3 sqrt(mid + sqrt(0.1)))."""
+    _, pre = _anisotropic_preprocess()
+    vis = pre["visible"]
+    a, b, c = pre["cov2D"][vis].unbind(1)
+    A = torch.stack([a, b, b, c], 1).reshape(-1, 2, 2).numpy()
+    lam_max = np.linalg.eigvalsh(A)[:, 1]
+    gap = (a - c) ** 2 / 4 + b * b
+    wide = (gap >= 0.1).numpy()
+    assert wide.sum() > 100 and (~wide).sum() >= 5
+    r = pre["radius_f"][vis].numpy()
+    np.testing.assert_array_equal(r[wide], np.ceil(3 * np.sqrt(lam_max[wide])))
+    mid = ((a + c) / 2).numpy()
+    np.testing.assert_array_equal(r[~wide], np.ceil(3 * np.sqrt(mid[~wide] + math.sqrt(0.1))))
+    # the radius is never below the largest eigenvalue's three sigma
+    assert (r >= 3 * np.sqrt(lam_max) - 1e-9).all()
+
+
+def test_radius_floor_special_case():
+    """A point-like Gaussian far away has cov2D = 0.3 I (the dilation only):
+    mid^2 - det = 0, the floor gives lambda = 0.3 + sqrt(0.1) and radius
+    ceil(3 sqrt(0.6162)) = 3, where the true eigenvalue 0.3 would give 2."""
+    st = _settings(64, 64, fov=60.0)
+    pre = OR.preprocess(torch.tensor([[0.0, 0.0, 40.0]]), torch.full((1, 1), 0.5), st,
+                        colors_precomp=torch.ones(1, 3), scales=torch.full((1, 3), 1e-7),
+                        rotations=torch.tensor([[1.0, 0, 0, 0]]))
+    assert torch.allclose(pre["cov2D"][0], torch.tensor([0.3, 0.0, 0.3], dtype=F64), atol=1e-12)
+    assert int(pre["radii"][0]) == 3
+    assert math.ceil(3 * math.sqrt(0.3)) == 2
+
+
 # -------------------------------------------------------------- backward ---
 
 def _grads_hand(state, st, means, dL, kw):
