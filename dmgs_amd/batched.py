@@ -406,7 +406,20 @@ class BatchedStep:
 
     ``run()`` launches eagerly; ``capture()`` records the same launches in a
     HIP graph and ``replay()`` re-issues them (the scene tensor, targets and
-    views are fixed at capture; update ``params`` in place between replays)."""
+    views are fixed at capture; update ``params`` in place between replays).
+
+    Instance capacity: the lists are sized once by ``plan`` (with headroom).
+    As the parameters change (Gaussians grow, move closer) a step can need
+    more instances than planned; its lists are then truncated and its images
+    and gradients are wrong.  Every step therefore copies the instance count
+    and the status word into pinned host memory behind its kernels, and
+    ``check()`` — called by ``run`` / ``replay`` unless ``check=False`` —
+    waits for them and, if the step overflowed (or, with view-major lists,
+    met a depth those keys cannot order), re-plans the capacity from the
+    step's own count (or switches to global cell lists), re-captures the
+    graph if there is one and runs the step again, so the step's outputs are
+    correct when ``check`` returns.  With ``check=False`` (timed loops) call
+    ``check()`` before using a step's outputs."""
 
     def __init__(self, renderer, loss, params):
         self.r, self.loss, self.params = renderer, loss, params
@@ -414,25 +427,77 @@ class BatchedStep:
         self.dimages = renderer.new_images()
         self.grads = torch.empty(CHANNELS, renderer.P, dtype=torch.float32, device=renderer.device)
         self.graph = None
+        self.replans = 0
+        self._status = torch.zeros(2, dtype=torch.int32, pin_memory=True)  # instances, status word
+        self._status_event = torch.cuda.Event()
+        self._status_pending = False
 
-    def run(self):
+    def _launch(self):
         self.r.forward(self.params, self.images)
         self.loss.forward(self.images)
         self.loss.backward(self.images, self.dimages)
         self.r.backward(self.params, self.dimages, self.grads)
+
+    def _record_status(self):
+        # behind the step on the same stream, outside any graph
+        self._status[0:1].copy_(self.r.instance_count(), non_blocking=True)
+        o = self.r._offsets[8]
+        self._status[1:2].copy_(self.r.state[o:o + 4].view(torch.int32), non_blocking=True)
+        self._status_event.record()
+        self._status_pending = True
+
+    def run(self, check=True):
+        self._launch()
+        self._record_status()
+        if check:
+            self.check()
         return self.loss.losses
 
     def capture(self):
-        self.run()  # warm the kernels and the allocator outside capture
+        self._launch()  # warm the kernels and the allocator outside capture
+        self._record_status()
         torch.cuda.synchronize(self.r.device)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            self.run()
+            self._launch()
         self.graph = g
         return g
 
-    def replay(self):
+    def replay(self, check=True):
         if self.graph is None:
             raise RuntimeError("capture() first")
         self.graph.replay()
+        self._record_status()
+        if check:
+            self.check()
         return self.loss.losses
+
+    def overflowed(self):
+        """Whether the last step must be redone (waits for its status)."""
+        if not self._status_pending:
+            return False
+        self._status_event.synchronize()
+        K = int(self._status[0].item()) & 0xFFFFFFFF
+        far = self.r.cell_mode == 1 and (int(self._status[1].item()) & 1) != 0
+        return far or K > self.r.K_cap
+
+    def check(self):
+        """Wait for the last step; if it overflowed, re-plan and redo it (see
+        the class docstring).  Returns True when the step was redone."""
+        redone = False
+        while self.overflowed():
+            K = int(self._status[0].item()) & 0xFFFFFFFF
+            if self.r.cell_mode == 1 and (int(self._status[1].item()) & 1) != 0:
+                self.r.cell_mode = 0
+            if K > self.r.K_cap:
+                self.r.plan(self.params, K_cap=max(1024, int(math.ceil(K * self.r.headroom))))
+            self.replans += 1
+            redone = True
+            if self.graph is not None:
+                self.graph = None
+                self.capture()  # its eager launch is the redone step
+            else:
+                self._launch()
+                self._record_status()
+        self._status_pending = False
+        return redone

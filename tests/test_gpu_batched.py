@@ -12,6 +12,7 @@
 * A captured HIP graph of the whole step reproduces the eager step on every
   replay, with eager work and synchronisation between replays.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -447,3 +448,69 @@ def test_densification_stats_match_per_view_consumer():
     scale = m.xyz_gradient_accum.abs().max().item()
     print(f"max |accum diff| {err:.3g} of {scale:.3g}", flush=True)
     assert scale > 0 and err <= 1e-4 * scale
+
+
+def test_training_steps_replan_on_overflow():
+    """Five captured training steps with the parameters updated in place
+    between replays (every log-scale grows by 0.25 per step, so the instance
+    count outgrows the planned capacity): each step's check re-plans and
+    redoes the step, and every step's images, losses and gradients equal an
+    eager step of a renderer planned for exactly those parameters."""
+    print("case: training steps with capacity re-planning", flush=True)
+    model = S.free_scene(20000, seed=21, layout="mixture", shell_radius=3.0)
+    sizes = [(160, 120), (128, 128), (96, 64)]
+    cams = _cameras(sizes, seed=23)
+    bgs = _bgs(3)
+    r, params = _batched(model, cams, bgs)
+    targets = [S.target_image(h, w, seed=40 + i) for i, (w, h) in enumerate(sizes)]
+    step = BT.BatchedStep(r, BT.BatchedLoss(r, targets), params)
+    step.capture()
+    caps = [r.K_cap]
+    for it in range(5):
+        with torch.no_grad():
+            params[BT.SCALE:BT.SCALE + 3] += 0.25
+        losses = step.replay().clone()
+        torch.cuda.synchronize()
+        assert not step.overflowed() and not r.overflowed()
+        caps.append(r.K_cap)
+        fresh = BT.BatchedRenderer(cams, [b.cuda() for b in bgs], model.num, sh_degree=3)
+        fresh.plan(params)
+        ref = BT.BatchedStep(fresh, BT.BatchedLoss(fresh, targets), params)
+        ref_losses = ref.run().clone()
+        torch.cuda.synchronize()
+        assert torch.equal(step.images, ref.images), f"step {it}: images differ"
+        assert torch.equal(losses, ref_losses), f"step {it}: losses differ"
+        assert grad_rel_err(step.grads, ref.grads) < 1e-5, f"step {it}: gradients differ"
+        print(f"  step {it}: instances {int(r.instance_count().item())}, capacity {r.K_cap}, re-plans "
+              f"{step.replans}", flush=True)
+    assert step.replans >= 2 and caps[-1] > caps[0]
+
+
+def test_65536_cells_keep_one_run_per_cell():
+    """65535 views of 16x16 pixels and one of 64x64: 65536 binning cells,
+    global cell lists.  The sort must not reach the first tile-mask bit above
+    the 16-bit cell id, or the big view's cell would split into two runs:
+    the big view and two small ones equal the drop-in single-view render."""
+    print("case: 65536 cells (65535 views of 16x16 + one of 64x64)", flush=True)
+    model = S.free_scene(48, seed=31, layout="uniform")
+    pos = np.array([0.3, -0.2, 4.0])
+    small = C.camera_at(pos, 16, 16, fov_deg=50.0, device="cpu")
+    big = C.camera_at(pos, 64, 64, fov_deg=50.0, device="cpu")
+    cams = [small] * 65535 + [big]
+    bg = torch.tensor([0.2, 0.3, 0.4])
+    r = BT.BatchedRenderer(cams, bg.cuda(), model.num, sh_degree=3)
+    assert r.total_cells == 65536 and r.cell_mode == 0
+    params = BT.pack_model(model.to("cuda"))
+    r.plan(params, backward=False)
+    assert r.lib.dmgs_batched_cell_sort_bits(ctypes.byref(r._desc())) == 16
+    images = r.forward(params)
+    torch.cuda.synchronize()
+    assert not r.overflowed()
+    m = model.to("cuda")
+    with torch.no_grad():
+        for v, cam in ((65535, big), (0, small), (40000, small)):
+            ref = render(cam, m, PipelineParams(), bg.cuda())["render"]
+            amb = _ambiguous(model, cam, bg).to("cuda")
+            err = (r.image(images, v) - ref).abs()[:, ~amb].max().item()
+            print(f"  view {v}: max image error {err:.3g}", flush=True)
+            assert err < IMG_TOL
