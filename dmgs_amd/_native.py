@@ -92,6 +92,7 @@ _SIGNATURES = {
                                                      c_int32, c_float, c_void_p, c_void_p, c_void_p]),
     "dmgs_scan_scratch_size": (c_size_t, [c_int64]),
     "dmgs_inclusive_scan_u32": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    "dmgs_inclusive_scan_u32_noalloc": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p]),
     "dmgs_batched_prepare_views": (c_int32, [ctypes.POINTER(DmgsView), c_int32, ctypes.POINTER(c_int32),
                                              ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
                                              ctypes.POINTER(c_int64)]),

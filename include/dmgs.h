@@ -150,6 +150,16 @@ int dmgs_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int6
 size_t dmgs_scan_scratch_size(int64_t n);
 int dmgs_inclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, void* scratch, dmgs_stream_t stream);
 
+/* dmgs_inclusive_scan_u32 without a scratch argument: the block sums live in
+ * a static device array of the library (1 MiB per GPU, held until the
+ * process ends), so the call allocates nothing and takes no workspace.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  n <= 2^30 (larger n returns DMGS_EINVAL); the stream
+ * must belong to the current device.  Calls that overlap on one GPU share the
+ * array and race: serialise them. */
+int dmgs_inclusive_scan_u32_noalloc(const uint32_t* in, uint32_t* out, int64_t n, dmgs_stream_t stream);
+
 /* ---- mesh -> Gaussian binding ------------------------------------------------ */
 /* Shared arguments: verts (V,3) float32, faces (F,3) int32, G Gaussians per
  * face in face-major order (Gaussian f*G + j), barycentrics bc either (G,3)

@@ -58,6 +58,12 @@ def test_argument_validation_without_gpu():
     assert "sh_degree" in N.last_error()
     assert lib.dmgs_sort_pairs(fake, fake, fake, fake, 10, 40, fake, None) == -1
     assert lib.dmgs_inclusive_scan_u32(None, None, 5, None, None) == -1
+    # the allocation-free scan checks its size limit and pointers before any HIP call
+    assert lib.dmgs_inclusive_scan_u32_noalloc(fake, fake, (1 << 30) + 1, None) == -1
+    assert "maximum of 1073741824" in N.last_error()
+    assert lib.dmgs_inclusive_scan_u32_noalloc(None, fake, 5, None) == -1
+    assert lib.dmgs_inclusive_scan_u32_noalloc(fake, fake, -1, None) == -1
+    assert lib.dmgs_inclusive_scan_u32_noalloc(None, None, 0, None) == 0
     assert lib.dmgs_dssim_l1_forward(fake, fake, 0, 4, 4, 0.2, fake, fake, None) == -1
     assert "image shape" in N.last_error()
 
