@@ -59,6 +59,10 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=200,
                     help="forward render requests for the serving figures (0 disables)")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--pair-tiles", type=int, default=64,
+                    help="tiles of view 0 on which the pair evaluations are counted")
+    ap.add_argument("--uneven-steps", type=int, default=8,
+                    help="graph replays of the uneven 3M / 16-view workload (0 disables; default workload only)")
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection mode: a few eager batched steps plus a 256 MiB device copy of known "
                          "size (the scale for FETCH_SIZE / WRITE_SIZE), then exit; run under rocprofv3 --pmc")
@@ -227,7 +231,7 @@ def run_plain(model, cams, targets, bg, steps, warmup):
             "stage_ms": {k: round(v, 4) for k, v in timer.averages_ms().items()}}
 
 
-def serving_run(n_requests, rate, sweep_rates=(100.0, 400.0, 1600.0)):
+def serving_run(n_requests, rate, sweep_rates=(100.0, 400.0, 1600.0, 6400.0)):
     """Four resident scenes (0.5M / 1M / 2M / 3M Gaussians, same
     distributions), Poisson arrivals at ``rate``, random scene, camera on a
     sphere of radius U[3, 6], resolution drawn from 256^2 .. 1920x1080;
@@ -256,6 +260,11 @@ def serving_run(n_requests, rate, sweep_rates=(100.0, 400.0, 1600.0)):
                                          "mean_batch") if k in st})
         sweep[-1]["rate_per_s"] = r
     stats["sweep"] = sweep
+    # capacity: the same number of requests all arriving at once (the queue
+    # is never empty, so renders/s is what the server sustains)
+    st = SV.serve_once(srv, SV.poisson_requests(names, n_requests, None, seed=77))
+    stats["capacity"] = {k: st[k] for k in ("p50_ms", "p99_ms", "renders_per_s", "mean_batch") if k in st}
+    stats["latency_clock"] = "from each request's scheduled arrival"
     return stats
 
 
@@ -326,6 +335,54 @@ def cpu_baseline(model, cam, budget_s):
                        f"instance count to {t_blend:.1f}s; loss excluded")}
 
 
+def cpu_small_view(repeats=5):
+    """FP64 CPU oracle, full forward + backward of the small workload (10k
+    free Gaussians, one 200x200 view, L1 against a seeded target): median of
+    ``repeats`` runs."""
+    from dmgs_amd import cameras, scenes
+    from oracle import raster as OR
+    m = scenes.free_scene(10_000, seed=0, layout="uniform")
+    cam = cameras.front_camera(200, 200, device="cpu")
+    st = OR.settings_for(cam, torch.ones(3), 3)
+    target = scenes.target_image(200, 200, seed=1).double()
+    kw = dict(shs=m.get_features, scales=m.get_scaling, rotations=m.get_rotation)
+    times = []
+    for _ in range(repeats):
+        a = time.perf_counter()
+        col, _, state = OR.rasterize(m.get_xyz, m.get_opacity, st, **kw)
+        dL = torch.sign(col - target) / col.numel()
+        OR.rasterize_backward(state, st, m.get_xyz, dL, **kw)
+        times.append(time.perf_counter() - a)
+    t = float(np.median(times))
+    return {"value": round(1.0 / t, 4), "unit": "it/s", "seconds_per_it": round(t, 3), "cores": torch.get_num_threads(),
+            "kind": "port", "sample": f"10000 Gaussians, one 200x200 view, full fwd+bwd of L1, median of {repeats}"}
+
+
+def cpu_mesh_view():
+    """FP64 CPU oracle, one full forward + backward of the mesh-bound
+    workload (81 920 faces x 6 Gaussians, one 800x800 view) into the vertices."""
+    from dmgs_amd import binding as PB
+    from dmgs_amd import cameras
+    from oracle import binding as OB
+    from oracle import raster as OR
+    gm = PB.mesh_scene(subdivisions=6, seed=0)
+    st = OR.settings_for(cameras.front_camera(800, 800, device="cpu"), torch.ones(3), 3)
+    feats = torch.cat([gm._features_dc, gm._features_rest], 1)
+    a = time.perf_counter()
+    v = gm.verts.double().requires_grad_(True)
+    xyz, q, s, _ = OB.stage3(v, gm.faces, gm.bc_coords.double(), gm._rotation.double(), gm._scaling.double(),
+                             gm.thin_z_scale)
+    kw = dict(shs=feats, scales=s.detach(), rotations=q.detach())
+    col, _, state = OR.rasterize(xyz.detach(), torch.sigmoid(gm._opacity), st, **kw)
+    dL = torch.full_like(col, 1.0 / col.numel())
+    g = OR.rasterize_backward(state, st, xyz.detach(), dL, **kw)
+    torch.autograd.backward([xyz, q], [g[0], g[6]])
+    t = time.perf_counter() - a
+    return {"value": round(1.0 / t, 5), "unit": "it/s", "seconds_per_it": round(t, 2), "cores": torch.get_num_threads(),
+            "kind": "port", "sample": f"{xyz.shape[0]} mesh-bound Gaussians, one 800x800 view, one full fwd+bwd "
+                                      "through the binding into the vertices"}
+
+
 def _cpu_cam(cam):
     class C:
         pass
@@ -392,6 +449,164 @@ def probe(args, model, cams, targets, bg):
     log(f"probe done: {args.steps} eager steps, instances {int(r.instance_count().item())}, copy 268435456 B x3")
 
 
+def measure_batched(model, cams, targets, bg, steps, warmup, autotune=True):
+    """Batched fwd+bwd of every view in one HIP graph replay per step:
+    throughput, stage times (device events on the launching stream over
+    eager steps of the same work) and the step's sizes."""
+    from dmgs_amd import batched as BT
+    params = BT.pack_model(model)
+    r = BT.BatchedRenderer(cams, bg, model.num, sh_degree=3)
+    K_cap = r.plan(params)
+    loss = BT.BatchedLoss(r, targets, 0.2)
+    step = BT.BatchedStep(r, loss, params)
+    tune = r.autotune(params, reps=2) if autotune else {}
+    log(f"planned K_cap={K_cap}; autotune ms {tune} -> variant {r.variant}")
+    step.capture()
+    for _ in range(warmup):
+        step.replay()
+    torch.cuda.synchronize()
+    log(f"timing {steps} batched steps ({len(cams)} views each)")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step.replay(check=False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if step.check() or r.overflowed():
+        raise RuntimeError("instance capacity overflowed during the timed region")
+    timer = StageTimer()
+    for _ in range(max(3, steps // 2)):
+        with timer("geometry+binning"):
+            r.forward(params, step.images, stages=BT.STAGE_GEOMETRY | BT.STAGE_BINNING)
+        with timer("render_forward"):
+            r.forward(params, step.images, stages=BT.STAGE_BLEND)
+        with timer("loss"):
+            loss.forward(step.images)
+            loss.backward(step.images, step.dimages)
+        with timer("render_backward"):
+            r.backward(params, step.dimages, step.grads, stages=BT.STAGE_BLEND)
+        with timer("geometry_backward"):
+            r.backward(params, step.dimages, step.grads, stages=BT.STAGE_GEOMETRY)
+    torch.cuda.synchronize()
+    K_cell, K_tile = list_counts(r)
+    return dict(r=r, params=params, step=step, loss=loss, tune=tune, K_cap=K_cap, elapsed=elapsed,
+                it_s=len(cams) * steps / elapsed, ms_step=1000.0 * elapsed / steps, stage_ms=timer.averages_ms(),
+                K=K_cell, K_tile=K_tile, V=int((r.radii() > 0).sum().item()), X=r.total_pixels, T=r.total_tiles)
+
+
+def list_counts(r):
+    """(cell instances, tile instances) of the last forward: a cell
+    instance's tile mask holds one bit per (splat, tile) pair of the
+    reference's per-tile lists."""
+    import ctypes
+    from dmgs_amd import _native as N
+    ko, vo = N.c_size_t(0), N.c_size_t(0)
+    N.check(r.lib.dmgs_batched_list_offsets(ctypes.byref(r._desc()), ctypes.byref(ko), ctypes.byref(vo)), "offsets")
+    K = min(int(r.instance_count().item()) & 0xFFFFFFFF, r.K_cap)
+    mask = (r.lists[ko.value:ko.value + 4 * K].view(torch.int32) >> 16) & 0xFFFF
+    bits = torch.zeros_like(mask)
+    for b in range(16):
+        bits += (mask >> b) & 1
+    return K, int(bits.sum().item())
+
+
+def _reach_strips(cx, cy, a, b, c, o, x0, y0):
+    """Which of the four 16x4 pixel strips of the tile at (x0, y0) each
+    splat's alpha >= 1/255 ellipse reaches, by the blend kernels' staging
+    test (same margins); (4, L) bool."""
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        tau = 2.0 * np.log(255.0 * o)
+    t = tau * 1.001 + 0.001
+    out = np.zeros((4, cx.size), dtype=bool)
+    m = 0.01
+    for k in range(4):
+        rx0, rx1 = x0 - m, x0 + 15.0 + m
+        ry0, ry1 = y0 + 4.0 * k - m, y0 + 4.0 * k + 3.0 + m
+        inside = (cx >= rx0) & (cx <= rx1) & (cy >= ry0) & (cy <= ry1)
+        best = np.full(cx.size, np.inf)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            for xe in (rx0, rx1):
+                dx = xe - cx
+                dy = np.clip(-b * dx / c, ry0 - cy, ry1 - cy)
+                best = np.minimum(best, a * dx * dx + 2 * b * dx * dy + c * dy * dy)
+            for ye in (ry0, ry1):
+                dy = ye - cy
+                dx = np.clip(-b * dy / a, rx0 - cx, rx1 - cx)
+                best = np.minimum(best, a * dx * dx + 2 * b * dx * dy + c * dy * dy)
+        out[k] = (tau > 0) & (inside | (best <= t) | ~((a > 0) & (c > 0)))
+    return out
+
+
+def pair_counts(r, view, n_tiles, seed=0):
+    """(pixel, splat) pair evaluations on sampled tiles of one view of the
+    last batched forward, from the device's own splat records and lists
+    (float64 restatement of the per-pixel rules):
+      reference:        a per-pixel loop, every list entry up to and including
+                        the pixel's stopping entry (the reference kernel's work);
+      strips_forward:   the one-wave-per-tile forward, 64 lanes per (splat,
+                        16x4 strip) the splat's footprint reaches while a pixel
+                        of the strip still blends;
+      strips_backward:  the strip backward, 64 lanes per (splat, reached strip)
+                        in front of the strip's last contributor;
+      contributing:     pairs that pass the alpha test before the pixel stops.
+    Also the sample's tile instances (list entries)."""
+    import ctypes
+    from dmgs_amd import _native as N
+    d = r.views[view]
+    W, H, gx, gy = d.width, d.height, d.grid_x, d.grid_y
+    cgx = (gx + 3) // 4
+    P = r.P
+    o5 = r._offsets[5]
+    rec = r.state[o5 + 48 * view * P:o5 + 48 * (view + 1) * P].view(torch.float32).view(P, 12).cpu().numpy()
+    rec = rec.astype(np.float64)
+    ko, vo = N.c_size_t(0), N.c_size_t(0)
+    N.check(r.lib.dmgs_batched_list_offsets(ctypes.byref(r._desc()), ctypes.byref(ko), ctypes.byref(vo)), "offsets")
+    o2 = r._offsets[2]
+    rng = np.random.default_rng(seed)
+    counts = dict(reference=0, strips_forward=0, strips_backward=0, contributing=0, tile_instances=0, tiles=0)
+    for t in rng.permutation(gx * gy)[:n_tiles]:
+        tx, ty = int(t) % gx, int(t) // gx
+        cell = d.cell_base + (ty // 4) * cgx + tx // 4
+        s0, s1 = r.state[o2 + 8 * cell:o2 + 8 * cell + 8].view(torch.int32).cpu().tolist()
+        keys = r.lists[ko.value + 4 * s0:ko.value + 4 * s1].view(torch.int32).cpu().numpy().view(np.uint32)
+        ids = r.lists[vo.value + 4 * s0:vo.value + 4 * s1].view(torch.int32).cpu().numpy().view(np.uint32)
+        bit = 1 << ((ty & 3) * 4 + (tx & 3))
+        g = ids[((keys >> 16) & bit) != 0].astype(np.int64) - view * P
+        L = g.size
+        counts["tiles"] += 1
+        counts["tile_instances"] += L
+        if L == 0:
+            continue
+        cx, cy, a, b, c, o = (rec[g, i] for i in (0, 1, 2, 3, 4, 5))
+        x0, y0 = 16.0 * tx, 16.0 * ty
+        reach = _reach_strips(cx, cy, a, b, c, o, x0, y0)
+        py, px = np.mgrid[0:16, 0:16]
+        px, py = (px + x0).reshape(-1), (py + y0).reshape(-1)
+        dx = cx[None, :] - px[:, None]
+        dy = cy[None, :] - py[:, None]
+        power = -0.5 * (a * dx * dx + c * dy * dy) - b * dx * dy
+        alpha = np.minimum(0.99, o * np.exp(power))
+        valid = (power <= 0) & (alpha >= 1.0 / 255.0)
+        Tinc = np.cumprod(np.where(valid, 1.0 - alpha, 1.0), axis=1)
+        stop = valid & (Tinc < 1e-4)
+        sidx = np.where(stop.any(1), stop.argmax(1), L)
+        j = np.arange(L)[None, :]
+        include = valid & (j < sidx[:, None])
+        ncontrib = np.where(include, j + 1, 0).max(1)
+        inside = (px < W) & (py < H)
+        counts["reference"] += int(np.minimum(sidx + 1, L)[inside].sum())
+        counts["contributing"] += int(include[inside].sum())
+        strip = (py - int(y0)) // 4
+        for k in range(4):
+            sel = inside & (strip == k)
+            if not sel.any():
+                continue
+            alive_to = sidx[sel].max()          # the strip blends up to its last stopping entry
+            counts["strips_forward"] += 64 * int((reach[k] & (np.arange(L) <= alive_to)).sum())
+            behind = ncontrib[sel].max()       # entries in front of the strip's last contributor
+            counts["strips_backward"] += 64 * int((reach[k] & (np.arange(L) < behind)).sum())
+    return counts
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -410,54 +625,28 @@ def main():
         probe(args, model, cams, targets, bg)
         return
 
-    # ---- batched: plan, autotune, capture, time graph replays ----
-    params = BT.pack_model(model)
-    r = BT.BatchedRenderer(cams, bg, N, sh_degree=3)
-    K_cap = r.plan(params)
-    loss = BT.BatchedLoss(r, targets, 0.2)
-    step = BT.BatchedStep(r, loss, params)
-    tune = r.autotune(params, reps=2)
-    log(f"planned K_cap={K_cap}; autotune ms {tune} -> variant {r.variant}")
-    step.capture()
-    for _ in range(args.warmup):
-        step.replay()
-    torch.cuda.synchronize()
-    log(f"timing {args.steps} batched steps ({B} views each)")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step.replay(check=False)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    if step.check() or r.overflowed():
-        raise RuntimeError("instance capacity overflowed during the timed region")
-    it_s = B * args.steps / elapsed
-    ms_step = 1000.0 * elapsed / args.steps
-    K = int(r.instance_count().item())
-    V = int((r.radii() > 0).sum().item())
-    X, T = r.total_pixels, r.total_tiles
-    losses = loss.losses[:, 0].sum().item()
-
-    # ---- stage timing of the same step, eager, with device events ----
-    timer = StageTimer()
-    for _ in range(max(3, args.steps // 2)):
-        with timer("geometry+binning"):
-            r.forward(params, step.images, stages=BT.STAGE_GEOMETRY | BT.STAGE_BINNING)
-        with timer("render_forward"):
-            r.forward(params, step.images, stages=BT.STAGE_BLEND)
-        with timer("loss"):
-            loss.forward(step.images)
-            loss.backward(step.images, step.dimages)
-        with timer("render_backward"):
-            r.backward(params, step.dimages, step.grads, stages=BT.STAGE_BLEND)
-        with timer("geometry_backward"):
-            r.backward(params, step.dimages, step.grads, stages=BT.STAGE_GEOMETRY)
-    torch.cuda.synchronize()
-    stage_ms = timer.averages_ms()
+    m = measure_batched(model, cams, targets, bg, args.steps, args.warmup)
+    r, params, step, loss = m["r"], m["params"], m["step"], m["loss"]
+    it_s, ms_step, stage_ms = m["it_s"], m["ms_step"], m["stage_ms"]
+    K, V, X, T = m["K"], m["V"], m["X"], m["T"]
     ab = algorithmic_bytes(B * N, V, K, X, T)
     kern = max(("render_forward", "render_backward"), key=lambda k: stage_ms[k])
     achieved = ab[kern] / (stage_ms[kern] * 1e-3) / 1e9
     step_bytes = ab["geometry"] + ab["binning"] + ab["render_forward"] + ab["render_backward"] + \
         ab["geometry_backward"]
+    losses = loss.losses[:, 0].sum().item()
+    # pair evaluations on sampled tiles of the first view, scaled to the step by tile instances
+    pc = pair_counts(r, 0, args.pair_tiles, seed=1)
+    scale = m["K_tile"] / max(pc["tile_instances"], 1)
+    pairs = {k: int(round(pc[k] * scale)) for k in ("reference", "strips_forward", "strips_backward", "contributing")}
+    valu = _valu_issue(f"batched_{kern}", stage_ms[kern], args)
+    pairs_note = {"sample": f"{pc['tiles']} random tiles of view 0 ({pc['tile_instances']} tile instances), "
+                            f"scaled by the step's {m['K_tile']} tile instances",
+                  "forward_evaluated_per_contributing": round(pc["strips_forward"] / max(pc["contributing"], 1), 3),
+                  "backward_evaluated_per_contributing": round(pc["strips_backward"] / max(pc["contributing"], 1), 3),
+                  "reference_evaluated_per_contributing": round(pc["reference"] / max(pc["contributing"], 1), 3)}
+    if valu:
+        pairs_note["backward_valu_per_64_evaluated"] = round(valu["instructions"] / (pairs["strips_backward"] / 64), 2)
 
     # ---- forward-only latency: batch, and the plain single view ----
     # the objects made so far move to the permanent GC generation, so a full
@@ -507,19 +696,23 @@ def main():
         "config": {"workload": desc, "gaussians": N, "views": B, "sizes": [f"{w}x{h}" for w, h in sizes],
                    "path": "batched",
                    "step": f"{B} views fwd+bwd in one HIP graph replay; value = views/s",
-                   "visible_entries": V, "instances": K, "instance_capacity": K_cap, "render_variant": r.variant,
-                   "summed_loss": round(losses, 6)},
+                   "visible_entries": V, "instances": K, "tile_instances": m["K_tile"],
+                   "instance_capacity": m["K_cap"], "render_variant": r.variant, "summed_loss": round(losses, 6)},
         "plain": plain,
         "vs_plain": round(it_s / plain["value"], 3) if plain else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": _read_traffic(f"batched_{kern}"),
                      "kernel": kern, "kernel_ms": round(stage_ms[kern], 4), "algorithmic_bytes": ab[kern],
-                     "note": "algorithmic bytes = 8T + 40K + 20X (+36V backward) with K the cell instances; "
-                             "measured traffic (rocprofv3 FETCH/WRITE, copy-calibrated) is far lower: the blend "
-                             "kernels are VALU-bound",
-                     "valu_issue": _valu_issue(f"batched_{kern}", stage_ms[kern], args)},
+                     "K_cell": K, "K_tile": m["K_tile"],
+                     "note": "algorithmic bytes = 8T + 40K + 20X (+36V backward) with K = K_cell, the (splat, "
+                             "64x64-pixel cell) instances the lists hold; with K = K_tile, the reference's "
+                             "(splat, 16x16 tile) pairs, the figure would be ~5.7x higher than the bytes that "
+                             "exist to move.  Measured traffic (rocprofv3 FETCH/WRITE, copy-calibrated) is below "
+                             "the algorithmic bytes: the blend kernels are bound by VALU issue",
+                     "valu_issue": valu,
+                     "pairs": dict(pairs, **pairs_note)},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "autotune_ms": {str(k): round(v, 4) for k, v in tune.items()},
+        "autotune_ms": {str(k): round(v, 4) for k, v in m["tune"].items()},
         "render_ms_p50": round(float(np.percentile(plat, 50)), 3),
         "render_ms_p99": round(float(np.percentile(plat, 99)), 3),
         "batch_render_ms_p50": round(float(np.percentile(blat, 50)), 3),
@@ -529,20 +722,55 @@ def main():
         "step_algorithmic_GBs": round(step_bytes / (ms_step * 1e-3) / 1e9, 2),
     }
     log(f"batched {it_s:.2f} it/s; plain {plain['value'] if plain else 'skipped'}; stages {stage_ms}")
+    log(f"pairs {result['roofline']['pairs']}")
+    del step, loss, r, m
+    torch.cuda.empty_cache()
+    if args.workload == "views" and args.uneven_steps > 0:
+        result["uneven"] = uneven_run(args, dev, bg)
+        log(f"uneven {result['uneven']}")
     result["binding"] = binding_run(dev)
     log(f"binding {result['binding']}")
     if args.serving_requests > 0:
         log(f"serving: {args.serving_requests} requests at {args.serving_rate}/s over 4 resident scenes")
-        del step, loss, r
         torch.cuda.empty_cache()
         result["serving"] = serving_run(args.serving_requests, args.serving_rate)
         log(f"serving {result['serving']}")
     if args.cpu_baseline_seconds > 0:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(model, cams[0], args.cpu_baseline_seconds)
+        log(f"cpu baseline (1M, 800x800, sampled) {result['cpu_baseline']}")
+        result["cpu_baseline"]["small_view"] = cpu_small_view()
+        log(f"cpu baseline (10k, 200x200, full) {result['cpu_baseline']['small_view']}")
+        result["cpu_baseline"]["mesh_view"] = cpu_mesh_view()
+        log(f"cpu baseline (mesh-bound, 800x800, full) {result['cpu_baseline']['mesh_view']}")
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
+
+
+def uneven_run(args, dev, bg):
+    """The uneven multi-resolution workload (3M Gaussians, 16 orbit views of
+    512^2 .. 1920x1080) as one HIP graph per step, beside its plain
+    one-view-per-launch figure."""
+    import copy
+    ua = copy.copy(args)
+    ua.workload, ua.gaussians, ua.views = "uneven", None, None
+    model, cams, targets, sizes, desc = build_workload(ua, dev)
+    log(f"uneven workload ready: {desc}")
+    m = measure_batched(model, cams, targets, bg, args.uneven_steps, 2)
+    out = {"value": round(m["it_s"], 3), "unit": "it/s", "ms_per_step": round(m["ms_step"], 3),
+           "steps": args.uneven_steps, "workload": desc, "sizes": [f"{w}x{h}" for w, h in sizes],
+           "instances": m["K"], "tile_instances": m["K_tile"], "visible_entries": m["V"],
+           "render_variant": m["r"].variant, "stage_ms": {k: round(v, 3) for k, v in m["stage_ms"].items()}}
+    del m
+    torch.cuda.empty_cache()
+    if not args.no_plain:
+        plain = run_plain(model, cams, targets, bg, max(2, args.uneven_steps // 4), 1)
+        out["plain"] = plain
+        out["vs_plain"] = round(out["value"] / plain["value"], 3)
+    del model, cams, targets
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

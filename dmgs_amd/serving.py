@@ -38,22 +38,32 @@ from . import cameras as C
 
 class Request:
     """One forward render: scene name, camera (reference Camera fields) and
-    background.  Filled in by the server: ``image`` (3, H, W), ``status``
-    ('done' / 'rejected'), timing and the batch it ran in."""
+    background, optionally a timeout in seconds from submission.  Filled in
+    by the server: ``image`` (3, H, W), ``status`` ('done', 'rejected',
+    'cancelled' or 'timed_out'; only 'done' carries an image), timing and the
+    batch it ran in."""
 
     _ids = itertools.count()
 
-    def __init__(self, scene, camera, bg=(1.0, 1.0, 1.0), arrival=None, scaling_modifier=1.0):
+    def __init__(self, scene, camera, bg=(1.0, 1.0, 1.0), arrival=None, scaling_modifier=1.0, timeout_s=None):
         self.id = next(self._ids)
         self.scene, self.camera = scene, camera
         self.scaling_modifier = float(scaling_modifier)
         self.bg = torch.as_tensor(bg, dtype=torch.float32)
         self.arrival = arrival
+        self.timeout_s = None if timeout_s is None else float(timeout_s)
         self.image = None
         self.status = "new"
         self.error = None
         self.t_submit = self.t_done = None
         self.batch_size = 0
+        self.cancel_requested = False
+
+    @property
+    def deadline(self):
+        if self.timeout_s is None or self.t_submit is None:
+            return None
+        return self.t_submit + self.timeout_s
 
     @property
     def latency_ms(self):
@@ -272,20 +282,61 @@ class RenderServer:
             self.variant_ms[b.vkey][b.r.variant].append(b.t_start.elapsed_time(b.t_end))
         for v, q in enumerate(b.requests):
             q.t_done = now
-            if error is None:
+            if error is not None:
+                q.status, q.error = "rejected", error
+            elif q.cancel_requested:
+                q.status, q.error = "cancelled", "cancelled while rendering"
+            elif q.deadline is not None and now > q.deadline:
+                q.status, q.error = "timed_out", f"finished {1000.0 * (now - q.deadline):.3f} ms after its deadline"
+            else:
                 q.image = b.r.image(b.images, v)
                 q.status = "done"
-            else:
-                q.status, q.error = "rejected", error
             self.completed.append(q)
         self.reserved -= b.reserved
         self.free_streams.append(b.stream)
+
+    # ---- cancellation and timeouts -------------------------------------------
+    def cancel(self, req):
+        """Cancel a request: a pending one leaves the queue at once; one whose
+        batch is on the GPU finishes with the batch (its batch-mates are
+        unaffected) and is then reported as cancelled, without an image.
+        Returns False if the request had already completed."""
+        if req.status == "pending":
+            self.pending.remove(req)
+            req.status, req.error = "cancelled", "cancelled before it was scheduled"
+            req.t_done = time.perf_counter()
+            self.completed.append(req)
+            return True
+        if req.status == "running":
+            req.cancel_requested = True
+            return True
+        return False
+
+    def _expire(self, now):
+        """Requests still queued past their deadline leave the queue as timed
+        out (requests already on the GPU are judged when they finish)."""
+        if not any(q.timeout_s is not None for q in self.pending):
+            return
+        keep = collections.deque()
+        for q in self.pending:
+            if q.deadline is not None and now > q.deadline:
+                q.status, q.error = "timed_out", "deadline passed while queued"
+                q.t_done = now
+                self.completed.append(q)
+            else:
+                keep.append(q)
+        self.pending = keep
 
     def _requeue(self, b):
         """Give back a counted batch that cannot get its list memory while
         every other batch waits too: its requests return to the head of the
         queue (their geometry pass is repeated later)."""
         for q in reversed(b.requests):
+            if q.cancel_requested:
+                q.status, q.error = "cancelled", "cancelled while waiting for list memory"
+                q.t_done = time.perf_counter()
+                self.completed.append(q)
+                continue
             q.status = "pending"
             self.pending.appendleft(q)
         self.reserved -= b.reserved
@@ -302,6 +353,7 @@ class RenderServer:
         self.active = still
         if len(self.active) > 1 and all(getattr(b, "waiting", False) for b in self.active):
             self._requeue(self.active.pop())  # newest waiting batch yields its memory
+        self._expire(time.perf_counter())
         self._admit()
         return len(self.active)
 
@@ -318,15 +370,18 @@ class RenderServer:
     # ---- open-loop driver ---------------------------------------------------
     def run(self, requests, rate=None):
         """Submit ``requests`` at their ``arrival`` offsets (seconds from the
-        start; all at once when ``rate`` is None and arrivals are unset) and
-        serve until all are done.  Returns the completed requests."""
+        start; all at once when arrivals are unset) and serve until all are
+        done.  Returns the completed requests.  A request's latency counts
+        from its scheduled arrival, so time the host loop spends before it
+        notices an arrival is part of the latency."""
         t0 = time.perf_counter()
         order = sorted(requests, key=lambda q: q.arrival or 0.0)
         i = 0
         while i < len(order) or self.active or self.pending:
             now = time.perf_counter() - t0
             while i < len(order) and (order[i].arrival or 0.0) <= now:
-                self.submit(order[i])
+                q = self.submit(order[i])
+                q.t_submit = t0 + (q.arrival or 0.0)
                 i += 1
             self.poll()
         torch.cuda.synchronize(self.device)
@@ -337,6 +392,10 @@ def latency_stats(done, wall_s=None):
     ok = [q for q in done if q.status == "done"]
     lat = np.array([q.latency_ms for q in ok]) if ok else np.zeros(0)
     out = {"completed": len(ok), "rejected": sum(q.status == "rejected" for q in done)}
+    for st in ("cancelled", "timed_out"):
+        n = sum(q.status == st for q in done)
+        if n:
+            out[st] = n
     if ok:
         out.update(p50_ms=round(float(np.percentile(lat, 50)), 3), p99_ms=round(float(np.percentile(lat, 99)), 3),
                    mean_batch=round(float(np.mean([q.batch_size for q in ok])), 2))

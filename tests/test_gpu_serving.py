@@ -123,3 +123,32 @@ def test_poisson_stream_statistics():
     assert stats["completed"] == 60 and stats["rejected"] == 0
     assert 0 < stats["p50_ms"] <= stats["p99_ms"]
     assert stats["renders_per_s"] > 0
+
+
+def test_cancel_and_timeout_leave_other_requests_intact():
+    """One stream, batches of two: a running and a queued request are
+    cancelled and a request with a zero timeout expires in the queue; every
+    other request completes with the image it gets when served alone."""
+    print("case: serving cancellation and timeouts", flush=True)
+    scenes = _packed(_scenes())
+    srv = SV.RenderServer(scenes, max_views=2, num_streams=1)
+    pos = C.sphere_positions(6, 3.8, 31)
+    reqs = [SV.Request("a" if i % 2 else "b", _cam(p, 128, 96)) for i, p in enumerate(pos)]
+    for q in reqs:
+        srv.submit(q)
+    srv.poll()
+    running = [q for q in reqs if q.status == "running"]
+    pending = [q for q in reqs if q.status == "pending"]
+    assert running and pending
+    assert srv.cancel(running[0]) and srv.cancel(pending[0])
+    late = srv.submit(SV.Request("b", _cam(pos[0], 64, 64), timeout_s=0.0))
+    done = srv.drain()
+    torch.cuda.synchronize()
+    status = {q.id: q.status for q in done}
+    assert status[running[0].id] == "cancelled" and status[pending[0].id] == "cancelled"
+    assert status[late.id] == "timed_out" and late.image is None
+    survivors = [q for q in reqs if q not in (running[0], pending[0])]
+    assert all(status[q.id] == "done" for q in survivors)
+    for q in survivors:
+        solo = SV.RenderServer(scenes, max_views=1, num_streams=1).run([SV.Request(q.scene, q.camera)])
+        assert torch.equal(solo[0].image, q.image), f"request {q.id} differs from its solo render"
