@@ -114,7 +114,9 @@ _lock = threading.Lock()
 
 
 def library_path():
-    return _build.library_path()
+    """The in-tree build, or the library named by DMGS_LIBRARY (the CPU
+    tests point it at the host-side AddressSanitizer build)."""
+    return os.environ.get("DMGS_LIBRARY") or _build.library_path()
 
 
 def load():

@@ -4,6 +4,7 @@ Every ``csrc/*.hip`` file is compiled with hipcc for gfx950 only and linked
 into one shared library exposing the C-ABI of ``include/dmgs.h``.
 
     python -m dmgs_amd.build            # build (always recompiles)
+    python -m dmgs_amd.build --asan     # host-side AddressSanitizer build (CPU checks only)
 """
 import concurrent.futures as cf
 import os
@@ -39,9 +40,15 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src, verbose):
-    obj = os.path.join(OBJDIR, os.path.basename(src)[:-4] + ".o")
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", CSRC, "-c", src, "-o", obj]
+# AddressSanitizer on the host code only (argument checks, workspace layout,
+# view preparation); the device code is built as usual
+ASAN_HOST_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+ASAN_DIR = os.path.join(LIBDIR, "asan")
+
+
+def _compile(src, verbose, objdir=OBJDIR, extra=()):
+    obj = os.path.join(objdir, os.path.basename(src)[:-4] + ".o")
+    cmd = [hipcc()] + HIPCC_FLAGS + list(extra) + ["-I", CSRC, "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -56,15 +63,25 @@ def library_path():
     return os.path.join(LIBDIR, LIBNAME)
 
 
-def build(verbose=False):
-    """Compile every HIP source for gfx950 and link the shared library."""
-    os.makedirs(OBJDIR, exist_ok=True)
+def asan_library_path():
+    return os.path.join(ASAN_DIR, LIBNAME)
+
+
+def build(verbose=False, asan=False):
+    """Compile every HIP source for gfx950 and link the shared library
+    (asan: a second build with AddressSanitizer on the host code, into
+    dmgs_amd/lib/asan/, loaded only by the CPU tests that check the C-ABI's
+    host side under the sanitizer)."""
+    objdir = os.path.join(ASAN_DIR, "obj") if asan else OBJDIR
+    extra = ASAN_HOST_FLAGS if asan else []
+    os.makedirs(objdir, exist_ok=True)
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
-    out = library_path()
+        objs = list(ex.map(lambda s: _compile(s, verbose, objdir, extra), srcs))
+    out = asan_library_path() if asan else library_path()
     tmp = out + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC"] + (["-Xarch_host", "-fsanitize=address"] if asan
+                                                                      else []) + ["-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -75,4 +92,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, asan="--asan" in sys.argv))
