@@ -172,70 +172,90 @@ __global__ __launch_bounds__(256) void mv_compact_kernel(MVArgs a, const uint32_
   }
 }
 
+// The backward is two kernels over the Gaussians, each reading only its plane
+// of the accumulator rows and few enough registers for several waves per
+// SIMD: mv_geometry_bwd_kernel (projection, EWA and 3D-covariance chain,
+// opacity, densification statistics; writes the position gradient of those
+// paths and the scale, rotation and opacity channels) and then
+// mv_sh_bwd_kernel (SH coefficients and the view-direction term of the
+// position gradient, added to what the first kernel wrote).
 __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.P) return;
-  MVGaussian G;
-  load_mv(a, g, G);
+  const size_t P = (size_t)a.P;
+  const size_t N = (size_t)a.B * P;
+  const float* pr = a.params;
+  auto ch = [&](int c) { return pr[(size_t)c * P + g]; };
+  const float3 p = make_float3(ch(DMGS_MV_XYZ), ch(DMGS_MV_XYZ + 1), ch(DMGS_MV_XYZ + 2));
+  const float3 logs = make_float3(ch(DMGS_MV_SCALE), ch(DMGS_MV_SCALE + 1), ch(DMGS_MV_SCALE + 2));
+  const float4 rq = make_float4(ch(DMGS_MV_ROT), ch(DMGS_MV_ROT + 1), ch(DMGS_MV_ROT + 2), ch(DMGS_MV_ROT + 3));
+  const float ol = ch(DMGS_MV_OPACITY);
+  // the activations exactly as load_mv computes them
+  const float m = a.scale_modifier;
+  const float3 sc = make_float3(m * expf(logs.x), m * expf(logs.y), m * expf(logs.z));
+  const float nrm = fmaxf(sqrtf(rq.x * rq.x + rq.y * rq.y + rq.z * rq.z + rq.w * rq.w), 1e-12f);
+  const float rq_inv = 1.0f / nrm;
+  const float4 q = make_float4(rq.x / nrm, rq.y / nrm, rq.z / nrm, rq.w / nrm);
+  const float o = sigmoidf(ol);
+  float R[9], c3[6];
+  quat_to_R(q, R);
+  cov3d_from_sR(sc, R, c3);
   float3 dmean = make_float3(0.f, 0.f, 0.f);
   float dc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  float dsh[48];
-#pragma unroll
-  for (int k = 0; k < 48; ++k) dsh[k] = 0.f;
   float dop = 0.f;
   bool any = false;
   // densification statistics (gaussian_model.py:405-407, train.py:116-118)
   float d_acc = 0.f, d_cnt = 0.f, d_maxr = 0.f;
   if (a.dens) {
     d_acc = a.dens[g];
-    d_cnt = a.dens[(size_t)a.P + g];
-    d_maxr = a.dens[2 * (size_t)a.P + g];
+    d_cnt = a.dens[P + g];
+    d_maxr = a.dens[2 * P + g];
   }
+  // the next view's radius and row are requested before this view is
+  // computed (rows of culled entries hold zeros and are skipped)
+  auto row = [&](int v, float2 (&r)[3]) {
+    const float2* gr = reinterpret_cast<const float2*>(a.gacc + gacc_index(N, (size_t)v * P + g, 0));
+    r[0] = gr[0];
+    r[1] = gr[1];
+    r[2] = gr[2];
+  };
+  int rad_next = a.radii[g];
+  float2 nx[3];
+  row(0, nx);
   for (int v = 0; v < a.B; ++v) {
-    const size_t i = (size_t)v * a.P + g;
-    const int radius = a.radii[i];
+    const int radius = rad_next;
+    const float2 r0 = nx[0], r1 = nx[1], r2 = nx[2];
+    if (v + 1 < a.B) {
+      rad_next = a.radii[(size_t)(v + 1) * P + g];
+      row(v + 1, nx);
+    }
     if (radius <= 0) continue;
     any = true;
-    const float3* gr = reinterpret_cast<const float3*>(a.gacc + i * DMGS_GACC_STRIDE);
-    const float3 r0 = gr[0], r1 = gr[1], r2 = gr[2];
-    const float4 g03 = make_float4(r0.x, r0.y, r0.z, r1.x);
-    const float4 g47 = make_float4(r1.y, r1.z, r2.x, r2.y);
-    const float g8 = r2.z;
     const ViewParams vp = view_params(const_ptr(a.views)[__builtin_amdgcn_readfirstlane(v)]);
     if (a.dens) {
-      d_acc += sqrtf(g03.x * g03.x + g03.y * g03.y);
+      d_acc += sqrtf(r0.x * r0.x + r0.y * r0.y);
       d_cnt += 1.0f;
       d_maxr = fmaxf(d_maxr, (float)radius);
     }
-    const float3 dm = ewa_backward(G.p, G.c3, vp, g03.x, g03.y, g03.z, g03.w, g47.x, dc);
-    float b[16];
-    float3 gm;
-    const float3 dd = sh_backward(a.sh_degree, G.sh, G.p, vp.campos, make_float3(g47.z, g47.w, g8), b, gm);
-    dmean.x += dm.x + dd.x;
-    dmean.y += dm.y + dd.y;
-    dmean.z += dm.z + dd.z;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      dsh[3 * k] += b[k] * gm.x;
-      dsh[3 * k + 1] += b[k] * gm.y;
-      dsh[3 * k + 2] += b[k] * gm.z;
-    }
-    dop += g47.y;
+    const float3 dm = ewa_backward(p, c3, vp, r0.x, r0.y, r1.x, r1.y, r2.x, dc);
+    dmean.x += dm.x;
+    dmean.y += dm.y;
+    dmean.z += dm.z;
+    dop += r2.y;
   }
   float3 dlogs = make_float3(0.f, 0.f, 0.f);
   float4 drq = make_float4(0.f, 0.f, 0.f, 0.f);
   if (any) {
     float3 ds;
     float4 dq;
-    cov3d_backward(dc, G.s, G.R, G.q, ds, dq);
+    cov3d_backward(dc, sc, R, q, ds, dq);
     // scale = exp(log-scale): d/dlog = dL/dscale * exp(log) (dL/dscale is taken w.r.t. mod * scale)
-    dlogs = make_float3(ds.x * expf(G.logs.x), ds.y * expf(G.logs.y), ds.z * expf(G.logs.z));
+    dlogs = make_float3(ds.x * expf(logs.x), ds.y * expf(logs.y), ds.z * expf(logs.z));
     // q = rq / |rq|: d/drq = (dq - q (q . dq)) / |rq|
-    const float qd = G.q.x * dq.x + G.q.y * dq.y + G.q.z * dq.z + G.q.w * dq.w;
-    drq = make_float4((dq.x - G.q.x * qd) * G.rq_inv, (dq.y - G.q.y * qd) * G.rq_inv,
-                      (dq.z - G.q.z * qd) * G.rq_inv, (dq.w - G.q.w * qd) * G.rq_inv);
+    const float qd = q.x * dq.x + q.y * dq.y + q.z * dq.z + q.w * dq.w;
+    drq = make_float4((dq.x - q.x * qd) * rq_inv, (dq.y - q.y * qd) * rq_inv, (dq.z - q.z * qd) * rq_inv,
+                      (dq.w - q.w * qd) * rq_inv);
   }
-  const size_t P = (size_t)a.P;
   if (a.dens) {
     a.dens[g] = d_acc;
     a.dens[P + g] = d_cnt;
@@ -253,9 +273,58 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
   put(DMGS_MV_ROT + 1, drq.y);
   put(DMGS_MV_ROT + 2, drq.z);
   put(DMGS_MV_ROT + 3, drq.w);
-  put(DMGS_MV_OPACITY, dop * G.o * (1.0f - G.o));
+  put(DMGS_MV_OPACITY, dop * o * (1.0f - o));
+}
+
+__global__ __launch_bounds__(256) void mv_sh_bwd_kernel(MVArgs a) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.P) return;
+  const size_t P = (size_t)a.P;
+  const size_t N = (size_t)a.B * P;
+  const float* pr = a.params;
+  auto ch = [&](int c) { return pr[(size_t)c * P + g]; };
+  const float3 p = make_float3(ch(DMGS_MV_XYZ), ch(DMGS_MV_XYZ + 1), ch(DMGS_MV_XYZ + 2));
+  float sh[48], dsh[48];
 #pragma unroll
-  for (int k = 0; k < 48; ++k) put(DMGS_MV_SH + k, dsh[k]);
+  for (int k = 0; k < 48; ++k) {
+    sh[k] = ch(DMGS_MV_SH + k);
+    dsh[k] = 0.f;
+  }
+  float3 dmean = make_float3(0.f, 0.f, 0.f);
+  auto col = [&](int v) {
+    const float* gc = a.gacc + gacc_index(N, (size_t)v * P + g, DMGS_GACC_GEO);
+    return make_float3(gc[0], gc[1], gc[2]);
+  };
+  int rad_next = a.radii[g];
+  float3 col_next = col(0);
+  for (int v = 0; v < a.B; ++v) {
+    const int radius = rad_next;
+    const float3 dcol = col_next;
+    if (v + 1 < a.B) {  // requested before this view is computed
+      rad_next = a.radii[(size_t)(v + 1) * P + g];
+      col_next = col(v + 1);
+    }
+    if (radius <= 0) continue;
+    const ViewParams vp = view_params(const_ptr(a.views)[__builtin_amdgcn_readfirstlane(v)]);
+    float b[16];
+    float3 gm;
+    const float3 dd = sh_backward(a.sh_degree, sh, p, vp.campos, dcol, b, gm);
+    dmean.x += dd.x;
+    dmean.y += dd.y;
+    dmean.z += dd.z;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      dsh[3 * k] += b[k] * gm.x;
+      dsh[3 * k + 1] += b[k] * gm.y;
+      dsh[3 * k + 2] += b[k] * gm.z;
+    }
+  }
+  float* gd = a.grads;
+  gd[(size_t)DMGS_MV_XYZ * P + g] += dmean.x;
+  gd[(size_t)(DMGS_MV_XYZ + 1) * P + g] += dmean.y;
+  gd[(size_t)(DMGS_MV_XYZ + 2) * P + g] += dmean.z;
+#pragma unroll
+  for (int k = 0; k < 48; ++k) gd[(size_t)(DMGS_MV_SH + k) * P + g] = dsh[k];
 }
 
 __global__ __launch_bounds__(256) void gather_rects_kernel(const uint2* __restrict__ rects,
@@ -300,6 +369,7 @@ void launch_mv_compact(const MVArgs& a, const uint32_t* depth_keys, uint32_t* ke
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s) {
   if (a.P <= 0 || a.B <= 0) return;
   hipLaunchKernelGGL(mv_geometry_bwd_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(mv_sh_bwd_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
 }
 
 }  // namespace dmgs

@@ -24,16 +24,23 @@
 // One 48-byte gather feeds both blend kernels.
 #define DMGS_SPLAT_F4 3
 
-// Per-Gaussian accumulator written by the blend backward with atomics:
-// 9 floats (36 B, read as three 12-byte vectors) per Gaussian, packed (the
-// rows are read and cleared as whole cache lines, so padding would cost
-// traffic).
-//   [0] dL/dmean2D.x (NDC)  [1] dL/dmean2D.y  [2] dL/dconic.a  [3] dL/dconic.b (half)
-//   [4] dL/dconic.c  [5] dL/dopacity  [6..8] dL/dcolour rgb
-#define DMGS_GACC_STRIDE 9
+// Per-Gaussian accumulator written by the blend backward with atomics: 9
+// values per Gaussian in two planes, so each consumer reads only its part
+// (the geometry backward the first, the SH backward the second):
+//   plane 1, 6 floats (24 B) per Gaussian: [0] dL/dmean2D.x (NDC)
+//     [1] dL/dmean2D.y  [2] dL/dconic.a  [3] dL/dconic.b (half)
+//     [4] dL/dconic.c  [5] dL/dopacity
+//   plane 2 (after n x 6 floats), 3 floats (12 B) per Gaussian: [6..8] dL/dcolour rgb
+// Value q of Gaussian id in an accumulator of n Gaussians is at gacc_index(n, id, q).
+#define DMGS_GACC_STRIDE 9  // floats per Gaussian, both planes
 #define DMGS_GACC_USED 9
+#define DMGS_GACC_GEO 6     // values in plane 1
 
 namespace dmgs {
+
+__host__ __device__ __forceinline__ size_t gacc_index(size_t n, size_t id, int q) {
+  return q < DMGS_GACC_GEO ? id * DMGS_GACC_GEO + (size_t)q : n * DMGS_GACC_GEO + id * 3 + (size_t)(q - DMGS_GACC_GEO);
+}
 
 // One camera of a multi-view launch (host-filled, copied to the device).
 struct ViewDesc {

@@ -128,7 +128,7 @@ struct MVArgs {
   uint32_t* depth_keys;      // (B*P), culled entries left unwritten
   uint32_t* order;           // (B*P)
   uint32_t* wave_counts;     // mv_wave_count_words(P, B): visible entries per (view, block, wave)
-  float* gacc;               // (B*P, 16) blend-backward accumulator; zero between backward passes
+  float* gacc;               // B*P x DMGS_GACC_STRIDE blend-backward accumulator (gacc_index); zero between backward passes
   float* grads;              // (59, P)
   float* dens;               // optional (3, P): densification statistics (see dmgs_batch)
   uint32_t* status;          // optional: zeroed by the preprocess (flags set by the compaction)
@@ -170,7 +170,7 @@ struct RenderArgs {
   uint32_t* n_contrib;   // (H, W)
   // backward only
   const float* dL_dpix;  // (3, H, W)
-  float* gacc;           // (P, DMGS_GACC_STRIDE)
+  float* gacc;           // P x DMGS_GACC_STRIDE floats, two planes (gacc_index)
   // optional (multi-view): work-group slot -> tile, heaviest tiles first
   const uint32_t* tile_order;
   // optional (multi-view, strip forward): per tile, the largest last

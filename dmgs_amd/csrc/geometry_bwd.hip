@@ -43,11 +43,12 @@ __device__ __forceinline__ void store_dsh(float* __restrict__ dsh, int M, const 
 __global__ __launch_bounds__(256) void geometry_bwd_kernel(GeometryBwdArgs a) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.P) return;
-  const float3* g = reinterpret_cast<const float3*>(a.gacc + (size_t)idx * DMGS_GACC_STRIDE);
-  const float3 r0 = g[0], r1 = g[1], r2 = g[2];
-  const float4 g03 = make_float4(r0.x, r0.y, r0.z, r1.x);
-  const float4 g47 = make_float4(r1.y, r1.z, r2.x, r2.y);
-  const float g8 = r2.z;
+  const float2* gg = reinterpret_cast<const float2*>(a.gacc + gacc_index(a.P, idx, 0));  // 8-byte aligned rows
+  const float2 r0 = gg[0], r1 = gg[1], r2 = gg[2];
+  const float* gc = a.gacc + gacc_index(a.P, idx, DMGS_GACC_GEO);
+  const float4 g03 = make_float4(r0.x, r0.y, r1.x, r1.y);
+  const float4 g47 = make_float4(r2.x, r2.y, gc[0], gc[1]);
+  const float g8 = gc[2];
   const float dmx = g03.x, dmy = g03.y, dqa = g03.z, dqb = g03.w, dqc = g47.x, dop = g47.y;
   if (a.dL_dmeans2D) {
     a.dL_dmeans2D[3 * idx] = dmx;

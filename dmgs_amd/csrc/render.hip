@@ -699,7 +699,7 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
       const float val = s_acc[slot][k] * grad_scale(k, ddelx_dx, ddely_dy);
       if (val != 0.f) {
         const uint32_t gid = __float_as_uint(L.rgb_id[slot].w);
-        unsafeAtomicAdd(a.gacc + (size_t)gid * DMGS_GACC_STRIDE + k, val);
+        unsafeAtomicAdd(a.gacc + gacc_index(a.P, gid, k), val);
       }
     }
   }
@@ -771,7 +771,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
     for (int i = threadIdx.x; i < cnt * DMGS_GACC_USED; i += NT) {
       const int slot = i / DMGS_GACC_USED, q = i - slot * DMGS_GACC_USED;
       const float val = s_acc[buf][slot][q] * grad_scale(q, ddelx_dx, ddely_dy);
-      if (val != 0.f) unsafeAtomicAdd(a.gacc + (size_t)s_gid[buf][slot] * DMGS_GACC_STRIDE + q, val);
+      if (val != 0.f) unsafeAtomicAdd(a.gacc + gacc_index(a.P, s_gid[buf][slot], q), val);
     }
   };
   int buf = 0, prev_n = 0;
