@@ -162,6 +162,51 @@ __device__ __forceinline__ float row_sums9(const float (&v)[DMGS_GACC_USED], uns
   return row_lane < 4u ? r[0] : (row_lane < 8u ? r[1] : e);
 }
 
+// Sums over a one-wave work-group of nine per-lane values, through LDS: each
+// lane stores its values 0-7 (red[q][lane]); lane (row r, row position p)
+// then adds value 2r + (p >> 3) of the eight lanes 8 (p & 7) .. 8 (p & 7) + 7
+// (two 16-byte reads, seven adds), and two quad butterflies plus a shift by
+// four lanes leave the sum over all 64 lanes in row positions 7 and 15.
+// Value 8 takes DPP row sums and two row broadcasts (lane 63 ends with its
+// total).  About 17 VALU instead of row_sums9's 28 plus its selects.
+struct WaveSum9 {
+  int q;       // the value this lane sums in the LDS step (0..7)
+  int src;     // the first of its eight source lanes
+  bool owner;  // row position 7 or 15: ends with the total of value q
+  bool last;   // lane 63: ends with the total of value 8
+};
+__device__ __forceinline__ WaveSum9 wave_sum9_lane(unsigned lane) {
+  const unsigned p = lane & 15u, r = lane >> 4;
+  return {(int)(2u * r + (p >> 3)), (int)(8u * (p & 7u)), (p & 7u) == 7u, lane == 63u};
+}
+// Rows named in row_mask add lane 15 (ctrl 0x142) or lane 31 (0x143) of the
+// wave; the other rows add zero.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float add_row_bcast(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+// acc[0..8] receive the totals by plain stores.  Called by the whole wave.
+__device__ __forceinline__ void wave_sums9(const float (&v)[DMGS_GACC_USED], float (*red)[DMGS_WAVE],
+                                           const WaveSum9& w, unsigned lane, float* acc) {
+  __syncthreads();  // the previous call's reads of red are done
+#pragma unroll
+  for (int q = 0; q < 8; ++q) red[q][lane] = v[q];
+  __syncthreads();
+  const float4 a = *reinterpret_cast<const float4*>(&red[w.q][w.src]);
+  const float4 b = *reinterpret_cast<const float4*>(&red[w.q][w.src + 4]);
+  float s = ((a.x + a.y) + (a.z + a.w)) + ((b.x + b.y) + (b.z + b.w));
+  s = DMGS_DPP_ADD(s, 0xB1);   // quad_perm [1,0,3,2]
+  s = DMGS_DPP_ADD(s, 0x4E);   // quad_perm [2,3,0,1]: each quad holds its four sources' sum
+  s = DMGS_DPP_ADD(s, 0x114);  // row_shr 4: position 7 (15) adds position 3 (11)
+  float e = DMGS_DPP_ADD(v[8], 0xB1);
+  e = DMGS_DPP_ADD(e, 0x4E);
+  e = DMGS_DPP_ADD(DMGS_DPP_ADD(e, 0x124), 0x128);  // row_ror 4, row_ror 8: every lane holds its row's sum
+  e = add_row_bcast<0x142, 0xA>(e);                  // rows 1, 3 add rows 0, 2
+  e = add_row_bcast<0x143, 0xC>(e);                  // rows 2, 3 add rows 0 + 1
+  if (w.owner) acc[w.q] = s;
+  if (w.last) acc[8] = e;
+}
+
 // Backward state of one pixel, walking its list back to front.  The reference
 // keeps per channel ch an accumulator acc_ch (colour behind the current splat)
 // and needs only  sum_ch (col_ch - acc_ch) g_ch  (g = dL/dpixel), so one
@@ -721,8 +766,10 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   __shared__ float s_acc[2][BATCH][DMGS_GACC_USED];
   __shared__ uint32_t s_gid[2][BATCH];
   __shared__ uint32_t s_wc[1];
+  __shared__ __align__(16) float s_red[8][DMGS_WAVE];  // one splat's per-lane values 0-7
   const TileCtx c = tile_ctx<16>(a);  // pixel k = 0 of this lane: (lx, ly), ly in 0..3
   const unsigned lane = threadIdx.x & 63;
+  const WaveSum9 wsum = wave_sum9_lane(lane);
   const float pxf = (float)c.px;
   const float ddelx_dx = 0.5f * (float)c.W, ddely_dy = 0.5f * (float)c.H;  // d pixel / d NDC
   float pyf[4];
@@ -765,7 +812,6 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
   }
   const int todo_total = min((int)(c.range.y - c.range.x), (int)m);
 
-  const unsigned row_lane = lane & 15;
   // one batch's accumulated sums -> one global atomic per (splat, value)
   auto flush = [&](int buf, int cnt) {
     for (int i = threadIdx.x; i < cnt * DMGS_GACC_USED; i += NT) {
@@ -851,8 +897,7 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
       const float s_tdx = s_t * dx;
       const float v[DMGS_GACC_USED] = {dmean_x(q, s_tdx, s_tdy), dmean_y(q, s_tdx, s_tdy), s_tdx * dx, s_tdy * dx,
                                        s_tdy2, s_t * __builtin_amdgcn_rcpf(q.w), c0, c1, c2};
-      const float mine = row_sums9(v, row_lane);
-      if (row_lane < DMGS_GACC_USED) atomicAdd(&s_acc[buf][j][row_lane], mine);
+      wave_sums9(v, s_red, wsum, lane, s_acc[buf][j]);
     }
     prev_n = n;
     buf ^= 1;
