@@ -285,7 +285,7 @@ class BatchedRenderer:
         return self.state[o:o + 4].view(torch.int32)
 
     def depth_clamped(self):
-        """True when the last view-major forward met a visible depth of 2^29
+        """True when the last view-major forward met a visible depth of 2^27
         or more (its lists may misorder such far splats).  Reads back."""
         o = self._offsets[8]
         return self.cell_mode == 1 and (int(self.state[o:o + 4].view(torch.int32).item()) & 1) != 0

@@ -69,9 +69,9 @@ int bits_for(uint32_t n) {
 // K-independent state of a batch: per-entry splats / keys / order / radii /
 // scan, the depth-sort scratch, per-tile ranges and per-pixel blend state.
 struct StateLayout {
-  size_t splats, keys_a, keys_b, order_a, order_b, rects, rects_sorted, radii, offsets, gscratch, wave_counts,
+  size_t splats, keys_a, keys_b, order_a, order_b, rects, rects_sorted, radii, offsets, gscratch, visible,
       status, tile_order, tile_bucket, bucket_hist, tile_work, ranges, final_T, n_contrib, total;
-  StateLayout(size_t N, size_t T, size_t X, size_t wave_words) {
+  StateLayout(size_t N, size_t T, size_t X) {
     size_t o = 0;
     splats = o; o += align_up(N * DMGS_SPLAT_F4 * 16);
     keys_a = o; o += align_up(N * 4);
@@ -83,7 +83,7 @@ struct StateLayout {
     radii = o; o += align_up(N * 4);
     offsets = o; o += align_up(N * 4);
     gscratch = o; o += align_up(std::max(dmgs::radix_scratch_words(N), dmgs::scan_scratch_words(N)) * 4);
-    wave_counts = o; o += align_up(wave_words * 4);
+    visible = o; o += align_up(4);  // visible entries of the last forward (the depth sort's length)
     status = o; o += align_up(4);
     tile_order = o; o += align_up(T * 4);
     tile_bucket = o; o += align_up(T);
@@ -140,8 +140,7 @@ struct Ctx {
   ListLayout L;
   explicit Ctx(const dmgs_batch* b)
       : N((size_t)b->num_views * b->num_gaussians),
-        S(N, (size_t)b->total_tiles, (size_t)b->total_pixels,
-          dmgs::mv_wave_count_words((int)b->num_gaussians, b->num_views)),
+        S(N, (size_t)b->total_tiles, (size_t)b->total_pixels),
         L((size_t)b->capacity), mode(b->cell_mode),
         local_bits(b->cell_mode == 1 ? std::max(1, bits_for((uint32_t)(b->max_view_cells - 1))) : 0),
         global_bits(std::max(1, bits_for((uint32_t)(b->total_cells - 1)))) {}
@@ -168,9 +167,9 @@ dmgs::MVArgs mv_args(const dmgs_batch* b, const Ctx& c, const float* params) {
   a.splats = at<float4>(b->state, c.S.splats);
   a.radii = at<int32_t>(b->state, c.S.radii);
   a.rects = at<uint2>(b->state, c.S.rects);
-  a.depth_keys = at<uint32_t>(b->state, c.S.keys_b);  // compacted into keys_a / order_a before the sort
+  a.depth_keys = at<uint32_t>(b->state, c.S.keys_a);  // sorted in place into keys_a / order_a
   a.order = at<uint32_t>(b->state, c.S.order_a);
-  a.wave_counts = at<uint32_t>(b->state, c.S.wave_counts);
+  a.view_major = b->cell_mode == 1 ? 1 : 0;
   a.gacc = static_cast<float*>(b->bwd_ws);
   a.dens = b->dens;
   return a;
@@ -246,8 +245,7 @@ int dmgs_batched_tile_view(const dmgs_view* views, int32_t B, int32_t* tile_view
 
 size_t dmgs_batched_state_size(int32_t B, int64_t P, int32_t total_tiles, int64_t total_pixels) {
   if (B <= 0 || P <= 0 || total_tiles <= 0 || total_pixels <= 0) return 0;
-  return StateLayout((size_t)B * P, (size_t)total_tiles, (size_t)total_pixels,
-                     dmgs::mv_wave_count_words((int)P, B)).total;
+  return StateLayout((size_t)B * P, (size_t)total_tiles, (size_t)total_pixels).total;
 }
 
 size_t dmgs_batched_lists_size(int64_t capacity) {
@@ -266,7 +264,7 @@ int dmgs_batched_offsets(int32_t B, int64_t P, int32_t total_tiles, int64_t tota
     return bfail(DMGS_EINVAL, "invalid batch sizes");
   if (!offsets) return bfail(DMGS_EINVAL, "offsets is null");
   const size_t N = (size_t)B * P;
-  StateLayout S(N, (size_t)total_tiles, (size_t)total_pixels, dmgs::mv_wave_count_words((int)P, B));
+  StateLayout S(N, (size_t)total_tiles, (size_t)total_pixels);
   ListLayout Ls((size_t)capacity);
   offsets[6] = Ls.tkeys_a;  // one-pass view-major sorts leave the lists at tkeys_b / vals_b (see cell_lists)
   offsets[7] = Ls.vals_a;
@@ -314,17 +312,14 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
   bool chunks_ready = false;  // the duplicate's chunk table was filled by this call's scan
   if (stages & DMGS_STAGE_GEOMETRY) {
     dmgs::MVArgs ap = a;
-    ap.status = status;  // zeroed by the preprocess, set by the compaction
+    ap.status = status;  // zeroed by the preprocess, set by the depth sort
     dmgs::launch_mv_preprocess(ap, st);
     uint32_t* gs = at<uint32_t>(b->state, c.S.gscratch);
-    // visible entries, in entry order, become the depth sort's input
-    const size_t nw = dmgs::mv_wave_count_words((int)b->num_gaussians, b->num_views);
-    dmgs::launch_inclusive_scan(a.wave_counts, a.wave_counts, nw, gs, st);
-    const uint32_t* V_dev = a.wave_counts + (nw - 1);
-    uint32_t* keys = at<uint32_t>(b->state, c.S.keys_a);
-    dmgs::launch_mv_compact(a, a.depth_keys, keys, a.order, b->cell_mode == 1, status, st);
-    dmgs::radix_sort_depth_keys(keys, a.order, at<uint32_t>(b->state, c.S.keys_b),
-                                at<uint32_t>(b->state, c.S.order_b), N, gs, st, V_dev);
+    // the visible entries, in entry order, sorted by their keys; culled entries drop out in the first pass
+    uint32_t* V_dev = at<uint32_t>(b->state, c.S.visible);
+    dmgs::radix_sort_entry_keys(a.depth_keys, a.order, at<uint32_t>(b->state, c.S.keys_b),
+                                at<uint32_t>(b->state, c.S.order_b), N, gs, st, V_dev,
+                                b->cell_mode == 1 ? status : nullptr);
     dmgs::launch_gather_rects(a.rects, a.order, at<uint2>(b->state, c.S.rects_sorted), offs, N, st, V_dev);
     if ((stages & DMGS_STAGE_BINNING) && b->capacity > 0 && b->lists) {
       // the binning follows in this call: the scan also fills the duplicate's chunk table

@@ -54,19 +54,26 @@ __device__ __forceinline__ void load_mv(const MVArgs& a, int g, MVGaussian& G) {
   cov3d_from_sR(G.s, G.R, G.c3);
 }
 
-// One wave's count of visible entries per view, at wave_counts[(v * blocks +
-// block) * 4 + wave]: scanned, they give every visible entry its slot in the
-// compacted, entry-ordered depth-sort input.
-__device__ __forceinline__ void count_visible(const MVArgs& a, int v, bool visible) {
-  const uint64_t bal = __ballot(visible);
-  if ((threadIdx.x & 63) == 0)
-    a.wave_counts[((size_t)v * gridDim.x + blockIdx.x) * 4 + (threadIdx.x >> 6)] =  // 256-lane work-groups
-        (uint32_t)__popcll(bal);
+// Depth sort key of a visible entry.  View-major keys: the top byte of a
+// positive float's bits is its exponent field over two, so a visible depth
+// d > 0.2 (>= 2^-3) has a top byte of at least 0x3E, and d < 2^27 keeps it at
+// most 0x4C: the top byte fits 4 bits (0..14) after subtracting 0x3E and the
+// freed 4 bits carry the view.  Sorting (view << 28 | (top - 0x3E) << 24 |
+// low 24 bits) orders by view, then by depth exactly as the float bits do.
+// A depth outside that range gets the far bucket (bits 24-27 all set, low
+// bits zero, so never kSortSkip), which the sort reports in bit 0 of its
+// status word.
+__device__ __forceinline__ uint32_t entry_sort_key(float depth, int view, bool view_major) {
+  const uint32_t bits = __float_as_uint(depth);
+  if (!view_major) return bits;
+  const uint32_t top = bits >> 24;
+  if (top < 0x3Eu || top - 0x3Eu > 14u) return ((uint32_t)view << 28) | kFarBucketBits;
+  return ((uint32_t)view << 28) | ((top - 0x3Eu) << 24) | (bits & 0xFFFFFFu);
 }
 
 __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a.status && g == 0) *a.status = 0u;  // the compaction sets its flags after this kernel
+  if (a.status && g == 0) *a.status = 0u;  // the depth sort sets its flags after this kernel
   const bool live = g < a.P;  // every lane stays for the per-view ballots
   MVGaussian G;
   if (live) load_mv(a, g, G);
@@ -75,10 +82,10 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
     const size_t i = (size_t)v * a.P + g;
     SplatGeom o;
     const bool visible = live && project_splat(G.p, G.c3, vp, o);
-    count_visible(a, v, visible);
     if (!live) continue;
-    if (!visible) {  // only the radius is read for a culled entry
+    if (!visible) {  // the radius and the sort's skip key are all that is read of a culled entry
       a.radii[i] = 0;
+      a.depth_keys[i] = kSortSkip;
       continue;
     }
     float3 dir = make_float3(G.p.x - vp.campos[0], G.p.y - vp.campos[1], G.p.z - vp.campos[2]);
@@ -92,83 +99,7 @@ __global__ __launch_bounds__(256) void mv_preprocess_kernel(MVArgs a) {
     sp[2] = make_float4(rgb.z, o.depth, __int_as_float(o.radius), 0.0f);
     a.radii[i] = o.radius;
     a.rects[i] = pack_rect(o.rmin, o.rmax);  // binning reads rectangles in depth order
-    a.depth_keys[i] = __float_as_uint(o.depth);
-  }
-}
-
-// Compact the visible entries, in entry order, into (depth key, entry id)
-// pairs: the depth sort then runs over the visible entries only, with the
-// same stable tie order as over all entries.  wave_counts holds the
-// inclusive scan of the preprocess's per-wave counts.
-//
-// View-major keys: the top byte of a positive float's bits is its exponent
-// field over two, so a visible depth d > 0.2 (>= 2^-3) has a top byte of at
-// least 0x3E and d < 2^29 keeps it at most 0x4D: the top byte fits 4 bits
-// after subtracting 0x3E and the freed 4 bits carry the view.  Sorting
-// (view << 28 | (top - 0x3E) << 24 | low 24 bits) orders by view, then by
-// depth exactly as the float bits do.  A larger depth is clamped to the
-// last bucket and reported in bit 0 of *status.
-__device__ __forceinline__ uint32_t view_major_key(uint32_t depth_bits, int view, bool& clamped) {
-  const uint32_t top = depth_bits >> 24;
-  uint32_t e = top >= 0x3Eu ? top - 0x3Eu : 0u;
-  clamped = top < 0x3Eu || e > 15u;
-  e = min(e, 15u);
-  return ((uint32_t)view << 28) | (e << 24) | (depth_bits & 0xFFFFFFu);
-}
-
-// Up to kCompactViews views: every radius, every wave offset and every
-// visible entry's depth key is requested before the first store (the stores
-// could alias the loads as far as the compiler knows, so a per-view loop waits
-// for one memory round trip per view).
-constexpr int kCompactViews = 16;
-
-__global__ __launch_bounds__(256) void mv_compact_kernel(MVArgs a, const uint32_t* __restrict__ depth_keys,
-                                                         uint32_t* __restrict__ keys_out,
-                                                         uint32_t* __restrict__ ids_out, int view_major,
-                                                         uint32_t* __restrict__ status) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = g < a.P;
-  const uint64_t lt = lanemask_lt();
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  auto wave_base = [&](int v) -> uint32_t {  // visible entries of all earlier waves (inclusive scan)
-    const size_t w = ((size_t)v * gridDim.x + blockIdx.x) * 4 + wave;  // 256-lane work-groups
-    return w == 0 ? 0u : a.wave_counts[w - 1];
-  };
-  auto emit = [&](int v, bool visible, uint32_t key, uint32_t base, uint64_t bal) {
-    if (!visible) return;
-    const size_t i = (size_t)v * a.P + g;
-    const uint32_t slot = base + (uint32_t)__popcll(bal & lt);
-    if (view_major) {
-      bool clamped;
-      key = view_major_key(key, v, clamped);
-      if (clamped) atomicOr(status, 1u);
-    }
-    keys_out[slot] = key;
-    ids_out[slot] = (uint32_t)i;
-  };
-  if (a.B <= kCompactViews) {
-    int rad[kCompactViews];
-    uint32_t base[kCompactViews], key[kCompactViews];
-#pragma unroll
-    for (int v = 0; v < kCompactViews; ++v) {
-      rad[v] = (v < a.B && live) ? a.radii[(size_t)v * a.P + g] : 0;
-      base[v] = v < a.B ? wave_base(v) : 0u;
-    }
-#pragma unroll
-    for (int v = 0; v < kCompactViews; ++v) key[v] = rad[v] > 0 ? depth_keys[(size_t)v * a.P + g] : 0u;
-#pragma unroll
-    for (int v = 0; v < kCompactViews; ++v) {
-      if (v >= a.B) break;
-      const bool visible = rad[v] > 0;
-      emit(v, visible, key[v], base[v], __ballot(visible));
-    }
-    return;
-  }
-  for (int v = 0; v < a.B; ++v) {
-    const size_t i = (size_t)v * a.P + g;
-    const bool visible = live && a.radii[i] > 0;
-    const uint64_t bal = __ballot(visible);
-    emit(v, visible, visible ? depth_keys[i] : 0u, wave_base(v), bal);
+    a.depth_keys[i] = entry_sort_key(o.depth, v, a.view_major != 0);
   }
 }
 
@@ -355,15 +286,6 @@ void launch_gather_rects(const uint2* rects, const uint32_t* order, uint2* rects
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s) {
   if (a.P <= 0 || a.B <= 0) return;
   hipLaunchKernelGGL(mv_preprocess_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a);
-}
-
-size_t mv_wave_count_words(int P, int B) { return (size_t)B * ((P + 255) / 256) * 4; }
-
-void launch_mv_compact(const MVArgs& a, const uint32_t* depth_keys, uint32_t* keys_out, uint32_t* ids_out,
-                       bool view_major, uint32_t* status, hipStream_t s) {
-  if (a.P <= 0 || a.B <= 0) return;
-  hipLaunchKernelGGL(mv_compact_kernel, dim3((a.P + 255) / 256), dim3(256), 0, s, a, depth_keys, keys_out, ids_out,
-                     view_major ? 1 : 0, status);
 }
 
 void launch_mv_geometry_bwd(const MVArgs& a, hipStream_t s) {

@@ -44,6 +44,11 @@ void launch_inclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t
 void launch_inclusive_scan_chunks(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch,
                                   uint32_t* chunk_first, uint32_t K, hipStream_t s);
 
+// Depth sort key of a culled entry (never a sort key of a visible one), and
+// the far-depth bucket bits of view-major keys.
+constexpr uint32_t kSortSkip = 0xFFFFFFFFu;
+constexpr uint32_t kFarBucketBits = 0x0F000000u;
+
 // Stable LSD radix sort of (key, value) pairs over bits [0, bits), in passes of
 // at most 8 bits.  Data starts in (k0, v0); (k1, v1) is the ping-pong buffer.
 // The number of passes is always even, so the result ends in (k0, v0).
@@ -57,10 +62,16 @@ void radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, si
 bool radix_sort_pairs_min_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
                                  uint32_t* scratch, hipStream_t s, const uint32_t* n_dev = nullptr,
                                  bool first_hist_ready = false);
-// radix_sort_pairs over 32-bit float-depth keys (4 passes): the three low
-// bytes are close to uniform and are counted with plain LDS atomics.
-void radix_sort_depth_keys(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, uint32_t* scratch,
-                           hipStream_t s, const uint32_t* n_dev = nullptr);
+// Depth sort of a batch's entries straight from the preprocess's keys: k0
+// holds n per-entry 32-bit keys, kSortSkip for culled entries.  The first of
+// the four passes drops the skipped keys and takes each entry's index as its
+// value, and stores the number of kept keys in *n_out, which bounds the
+// other passes; the sorted (key, entry) pairs end in (k0, v0).  flags
+// (optional, view-major keys only): bit 0 is set when a key lies in the
+// far-depth bucket.  The three low bytes are close to uniform and are
+// counted with plain LDS atomics.
+void radix_sort_entry_keys(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, uint32_t* scratch,
+                           hipStream_t s, uint32_t* n_out, uint32_t* flags);
 // Digit width of the first pass of radix_sort_pairs_min_passes, or -1 when
 // its blocks are not 4096 keys (the duplicate's chunks) for this n.
 int radix_min_passes_first_width(size_t n, int bits);
@@ -125,22 +136,15 @@ struct MVArgs {
   float4* splats;            // (B*P) records
   int32_t* radii;            // (B*P)
   uint2* rects;              // (B*P) tile rectangle, pack_rect; 0 when culled
-  uint32_t* depth_keys;      // (B*P), culled entries left unwritten
+  uint32_t* depth_keys;      // (B*P) depth sort keys (entry_sort_key), kSortSkip for culled entries
   uint32_t* order;           // (B*P)
-  uint32_t* wave_counts;     // mv_wave_count_words(P, B): visible entries per (view, block, wave)
   float* gacc;               // B*P x DMGS_GACC_STRIDE blend-backward accumulator (gacc_index); zero between backward passes
   float* grads;              // (59, P)
   float* dens;               // optional (3, P): densification statistics (see dmgs_batch)
-  uint32_t* status;          // optional: zeroed by the preprocess (flags set by the compaction)
+  uint32_t* status;          // optional: zeroed by the preprocess (flags set by the depth sort)
+  int view_major;            // depth keys also order by view (at most 16 views)
 };
 void launch_mv_preprocess(const MVArgs& a, hipStream_t s);
-size_t mv_wave_count_words(int P, int B);
-// After an inclusive scan of wave_counts: visible entries in entry order as
-// (depth key, entry id) pairs; their number is wave_counts[last].  With
-// view_major the keys also order by view (at most 16 views; depths of 2^29
-// and more share one bucket and set bit 0 of *status).
-void launch_mv_compact(const MVArgs& a, const uint32_t* depth_keys, uint32_t* keys_out, uint32_t* ids_out,
-                       bool view_major, uint32_t* status, hipStream_t s);
 // rects_sorted[i] = rects[order[i]], counts[i] = its number of binning cells
 // (0 for i >= *n_dev when given).
 void launch_gather_rects(const uint2* rects, const uint32_t* order, uint2* rects_sorted, uint32_t* counts, size_t n,

@@ -14,6 +14,8 @@
 // Ranking inside a block is wavefront-level: each wave64 owns a contiguous
 // 1024-key slice, finds lanes with equal digits with per-bit ballots and
 // keeps running per-digit counters in LDS, so ties keep input order.
+#include <utility>
+
 #include "dmgs_common.h"
 #include "dmgs_kernels.h"
 
@@ -182,11 +184,15 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t d, int nbits, bool vali
 // SPREAD: the digits are close to uniform (the low bytes of depth keys), so
 // per-wave counters with one plain LDS atomic per key rarely collide;
 // otherwise lanes with equal digits are grouped by ballots first.
-template <int ITEMS, bool SPREAD>
+// FIRST (the depth sort's first pass over the preprocess's per-entry keys):
+// keys equal to kSortSkip (culled entries) are not counted, and a key in the
+// view-major far-depth bucket (bits 24-27 all set) sets bit 0 of *flags.
+template <int ITEMS, bool SPREAD, bool FIRST = false>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t* __restrict__ keys, size_t n,
                                                                  int shift, int nbits,
                                                                  uint32_t* __restrict__ hist, uint32_t nb,
-                                                                 const uint32_t* __restrict__ n_dev) {
+                                                                 const uint32_t* __restrict__ n_dev,
+                                                                 uint32_t* __restrict__ flags = nullptr) {
   __shared__ uint32_t cnt[256];
   __shared__ uint32_t wcnt[SPREAD ? kSortWaves : 1][256];
   if (n_dev) n = min(n, (size_t)*n_dev);  // device-side length (blocks past it count nothing)
@@ -212,10 +218,15 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
     // and counted by one LDS atomic from the group's highest lane: skewed
     // digits (e.g. the exponent byte of depth keys) do not serialise 64
     // atomics.
+    bool far = false;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       const size_t idx = base + (size_t)i * kSortThreads + threadIdx.x;
-      const bool valid = idx < n;
+      bool valid = idx < n;
+      if constexpr (FIRST) {
+        valid = valid && k[i] != kSortSkip;
+        far = far || (valid && (k[i] & kFarBucketBits) == kFarBucketBits);
+      }
       const uint32_t d = (k[i] >> shift) & mask;
       if constexpr (SPREAD) {
         if (valid) atomicAdd(&wcnt[threadIdx.x >> 6][d], 1u);
@@ -223,6 +234,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
         const uint64_t peers = digit_peers(d, nbits, valid);
         if (valid && lane == 63 - __clzll(peers)) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
       }
+    }
+    if constexpr (FIRST) {
+      if (flags && __ballot(far) != 0 && lane == 0) atomicOr(flags, 1u);
     }
   }
   __syncthreads();
@@ -265,11 +279,15 @@ __global__ __launch_bounds__(kScanThreads) void radix_rowscan_kernel(uint32_t* _
   if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-template <int ITEMS>
+// FIRST: keys equal to kSortSkip are dropped, each value is its key's input
+// position (vin unused), and work-group 0 stores the number of keys kept in
+// *n_out.
+template <int ITEMS, bool FIRST = false>
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, size_t n, int shift, int nbits, const uint32_t* __restrict__ hist_excl,
-    const uint32_t* __restrict__ totals, uint32_t nb, const uint32_t* __restrict__ n_dev) {
+    const uint32_t* __restrict__ totals, uint32_t nb, const uint32_t* __restrict__ n_dev,
+    uint32_t* __restrict__ n_out = nullptr) {
   constexpr int kTile = kSortThreads * ITEMS;
   constexpr int kSlice = 64 * ITEMS;
   if (n_dev) n = min(n, (size_t)*n_dev);
@@ -293,6 +311,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     uint32_t all_total;
     const uint32_t digit_base = block_exclusive_scan<kSortWaves>(dg <= mask ? totals[dg] : 0u, s_wave, all_total);
     if (dg <= mask) gbase[dg] = digit_base + hist_excl[(size_t)dg * nb + blockIdx.x];
+    if constexpr (FIRST) {
+      if (n_out && blockIdx.x == 0 && threadIdx.x == 0) *n_out = all_total;
+    }
   }
 
   const size_t wbase = (size_t)blockIdx.x * kTile + (size_t)wave * kSlice;
@@ -316,7 +337,8 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   }
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const bool valid = (uint32_t)(i * 64) + (uint32_t)lane < rem;
+    bool valid = (uint32_t)(i * 64) + (uint32_t)lane < rem;
+    if constexpr (FIRST) valid = valid && key[i] != kSortSkip;
     const uint32_t d = (key[i] >> shift) & mask;
     const uint64_t peers = digit_peers(d, nbits, valid);
     const uint32_t before = wcnt[wave][d];
@@ -343,7 +365,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     lpos[i] = (uint32_t)kTile;
-    if ((uint32_t)(i * 64) + (uint32_t)lane < rem) {
+    bool valid = (uint32_t)(i * 64) + (uint32_t)lane < rem;
+    if constexpr (FIRST) valid = valid && key[i] != kSortSkip;
+    if (valid) {
       const uint32_t d = (key[i] >> shift) & mask;
       lpos[i] = dstart[d] + wcnt[wave][d] + rank[i];
     }
@@ -358,8 +382,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   __syncthreads();
   // coalesced write-out, keys then values: each digit's run is contiguous in
   // LDS and in memory
-  const size_t bbase = (size_t)blockIdx.x * kTile;
-  const uint32_t count = (uint32_t)min((size_t)kTile, n - bbase);
+  const uint32_t count = block_total;  // the keys this block sorts (all of its slice unless skipped ones)
   for (uint32_t i = threadIdx.x; i < count; i += kSortThreads) {
     const uint32_t d = sdig[i];
     const uint32_t pos = gbase[d] + (i - dstart[d]);
@@ -367,7 +390,13 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   }
   uint32_t val[ITEMS];
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) val[i] = lpos[i] < (uint32_t)kTile ? vw[i * 64 + lane] : 0u;
+  for (int i = 0; i < ITEMS; ++i) {
+    if constexpr (FIRST) {
+      val[i] = (uint32_t)wbase + (uint32_t)(i * 64 + lane);
+    } else {
+      val[i] = lpos[i] < (uint32_t)kTile ? vw[i * 64 + lane] : 0u;
+    }
+  }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i)
@@ -388,10 +417,18 @@ size_t radix_scratch_words(size_t n) {
   return 256 * nb + 256 + 64;
 }
 
+// Entry-key first pass (see radix_sort_entry_keys): when `entry` is set, pass 0
+// reads k0 as per-entry keys with kSortSkip holes and implicit values and
+// counts the kept keys into entry->n_out, which bounds the later passes.
+struct EntryPass {
+  uint32_t* n_out;
+  uint32_t* flags;
+};
+
 template <int ITEMS>
 static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
                          uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes,
-                         bool first_hist_ready = false, int spread_below = 0) {
+                         bool first_hist_ready = false, int spread_below = 0, const EntryPass* entry = nullptr) {
   constexpr size_t kTile = (size_t)kSortThreads * ITEMS;
   const uint32_t nb = (uint32_t)((n + kTile - 1) / kTile);
   int widths[4];
@@ -407,6 +444,19 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
   int shift = 0;
   for (int p = 0; p < passes; ++p) {
     const int w = widths[p];
+    if (p == 0 && entry) {
+      hipLaunchKernelGGL((radix_hist_kernel<ITEMS, true, true>), dim3(nb), dim3(kSortThreads), 0, s,
+                         (const uint32_t*)ka, n, shift, w, hist, nb, (const uint32_t*)nullptr, entry->flags);
+      hipLaunchKernelGGL(radix_rowscan_kernel, dim3(1u << w), dim3(kScanThreads), 0, s, hist, nb, totals);
+      hipLaunchKernelGGL((radix_scatter_kernel<ITEMS, true>), dim3(nb), dim3(kSortThreads), 0, s,
+                         (const uint32_t*)ka, (const uint32_t*)nullptr, kb, vb, n, shift, w, (const uint32_t*)hist,
+                         (const uint32_t*)totals, nb, (const uint32_t*)nullptr, entry->n_out);
+      n_dev = entry->n_out;  // the later passes sort the kept keys only
+      shift += w;
+      std::swap(ka, kb);
+      std::swap(va, vb);
+      continue;
+    }
     if (p == 0 && first_hist_ready) {
     } else if (shift + w <= spread_below) {
       hipLaunchKernelGGL((radix_hist_kernel<ITEMS, true>), dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka, n,
@@ -431,20 +481,21 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
 
 static void radix_dispatch(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, int bits,
                            uint32_t* scratch, hipStream_t s, const uint32_t* n_dev, int passes,
-                           bool first_hist_ready = false, int spread_below = 0) {
+                           bool first_hist_ready = false, int spread_below = 0, const EntryPass* entry = nullptr) {
   if (sort_items(n) == 16)
-    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready, spread_below);
+    radix_passes<16>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready, spread_below, entry);
   else
-    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready, spread_below);
+    radix_passes<4>(k0, v0, k1, v1, n, bits, scratch, s, n_dev, passes, first_hist_ready, spread_below, entry);
 }
 
-void radix_sort_depth_keys(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, uint32_t* scratch,
-                           hipStream_t s, const uint32_t* n_dev) {
+void radix_sort_entry_keys(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, size_t n, uint32_t* scratch,
+                           hipStream_t s, uint32_t* n_out, uint32_t* flags) {
   if (n == 0) return;
   // every pass counts with per-wave LDS atomics: even the top byte (view and
   // exponent bits) spreads over enough digits that they rarely collide
   // (measured 5 us per sort faster than ballot grouping on 5.9M keys)
-  radix_dispatch(k0, v0, k1, v1, n, 32, scratch, s, n_dev, 4, false, 32);
+  const EntryPass entry{n_out, flags};
+  radix_dispatch(k0, v0, k1, v1, n, 32, scratch, s, nullptr, 4, false, 32, &entry);
 }
 
 static int min_passes(int bits) { return bits <= 8 ? 1 : (bits <= 16 ? 2 : 4); }

@@ -323,7 +323,7 @@ typedef struct dmgs_batch {
    * 65536 cells).  1: view-major — the depth sort also orders by view and
    * instances are keyed by (view, cell within the view), one sort pass when
    * every view has at most 256 cells; needs num_views <= 16, max_view_cells
-   * <= 4096 and visible depths below 2^29 (a larger one sets bit 0 of the
+   * <= 4096 and visible depths below 2^27 (a larger one sets bit 0 of the
    * status word; its order among such far splats is then not guaranteed:
    * re-run with cell_mode 0). */
   int32_t cell_mode;

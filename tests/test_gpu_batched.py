@@ -95,8 +95,7 @@ def _batched(model_cpu, cams, bgs, variant=0, cell_mode=None):
 @pytest.mark.parametrize("n,sizes", [(20000, [(128, 96), (200, 200), (160, 90)]),
                                      (5000, [(64, 48)]),
                                      (30000, [(256, 256), (96, 160), (33, 17), (200, 120)]),
-                                     # more than 16 views: global cell lists and the per-view
-                                     # compaction loop
+                                     # more than 16 views: global cell lists and plain depth keys
                                      (8000, [(48, 40), (40, 48), (64, 32)] * 6 + [(24, 24), (56, 40)])])
 def test_batched_matches_single_view(n, sizes):
     print(f"case: batched vs single-view n={n} sizes={sizes}", flush=True)
@@ -386,7 +385,7 @@ def test_cell_lists_equal_reference_tile_lists(cell_mode, sizes):
 
 
 def test_view_major_far_depth_falls_back():
-    """A visible splat at depth >= 2^29 cannot be ordered by view-major keys:
+    """A visible splat at depth >= 2^27 cannot be ordered by view-major keys:
     the forward reports it, the renderer switches to global cell lists, and
     the re-rendered images equal those of a cell-mode-0 renderer."""
     print("case: view-major far depth fallback", flush=True)
@@ -397,7 +396,7 @@ def test_view_major_far_depth_falls_back():
     Rt = c.world_view_transform.detach().cpu().double()
     center, fwd = c.camera_center.detach().cpu().double(), Rt[:3, 2]
     with torch.no_grad():
-        model._xyz[0] = (center + 6e8 * fwd).float()  # 2^29 = 5.4e8
+        model._xyz[0] = (center + 6e8 * fwd).float()  # 2^27 = 1.3e8
         model._scaling[0] = math.log(4e7)
         model._opacity[0] = 5.0
     bgs = _bgs(len(cams))
