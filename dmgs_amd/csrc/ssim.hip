@@ -33,7 +33,7 @@ constexpr int kR = 5;                          // window radius
 constexpr int kPW = kTW + 2 * kR;              // 42 staged columns
 constexpr int kPH = kTH + 2 * kR;              // 42 staged rows
 constexpr int kPS = 44;                        // staged row stride (16-byte rows)
-constexpr int kHS = 40;                        // blurred row stride
+constexpr int kHS = 32;                        // blurred row stride (32 floats: one more work-group per CU)
 constexpr int kQuads = kTW / 4;                // horizontal tasks per row
 constexpr int kHTasks = kPH * kQuads;          // 336
 static_assert(kTW == 32 && kTH == 32, "the vertical pass maps 256 threads to 32 columns x 8 row quads");
