@@ -757,16 +757,17 @@ __global__ __launch_bounds__(SUB* SUB) void render_bwd_kernel(RenderArgs a) {
 // gradients, so each (tile, splat) pays for one wave reduction instead of
 // four.
 template <int BATCH, bool CELLS>
-__global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void render_bwd_strips_kernel(RenderArgs a) {
   constexpr int NT = 64;
   __shared__ SplatLDS<BATCH> L;
   // Two accumulator sets: a batch's global atomics are issued after the next
   // batch has been staged, so no staging load waits behind them (the memory
   // counter is in order) and they complete while that batch is computed.
-  __shared__ float s_acc[2][BATCH][DMGS_GACC_USED];
+  __shared__ __align__(16) float s_acc[2][BATCH][DMGS_GACC_USED];
   __shared__ uint32_t s_gid[2][BATCH];
   __shared__ uint32_t s_wc[1];
-  __shared__ __align__(16) float s_red[8][DMGS_WAVE];  // one splat's per-lane values 0-7
+  static_assert(BATCH * DMGS_GACC_USED >= 8 * DMGS_WAVE && (BATCH * DMGS_GACC_USED) % 4 == 0,
+                "one accumulator set holds a splat's 8 x 64 per-lane values");
   const TileCtx c = tile_ctx<16>(a);  // pixel k = 0 of this lane: (lx, ly), ly in 0..3
   const unsigned lane = threadIdx.x & 63;
   const WaveSum9 wsum = wave_sum9_lane(lane);
@@ -847,6 +848,9 @@ __global__ __launch_bounds__(64) void render_bwd_strips_kernel(RenderArgs a) {
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     flush(buf ^ 1, prev_n);  // the previous batch's atomics overlap this batch
     __syncthreads();
+    // the flushed set is free until the next batch zeroes it: it holds the
+    // per-lane values of the wave sums (wave_sums9) meanwhile
+    float(*const s_red)[DMGS_WAVE] = reinterpret_cast<float(*)[DMGS_WAVE]>(&s_acc[buf ^ 1][0][0]);
     uint32_t mask_next = n > 0 ? splat_mask(L, 0) : 0u;  // read one splat ahead
     for (int j = 0; j < n; ++j) {
       const uint32_t reach = __builtin_amdgcn_readfirstlane(mask_next);
