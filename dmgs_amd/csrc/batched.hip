@@ -207,6 +207,12 @@ __global__ __launch_bounds__(256) void mv_geometry_bwd_kernel(MVArgs a) {
   put(DMGS_MV_OPACITY, dop * o * (1.0f - o));
 }
 
+// One thread's 48 SH coefficients in a [48][256] LDS array.
+struct ShColumn {
+  const float* base;
+  __device__ __forceinline__ float operator[](int k) const { return base[k * 256]; }
+};
+
 __global__ __launch_bounds__(256) void mv_sh_bwd_kernel(MVArgs a) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.P) return;
@@ -215,12 +221,16 @@ __global__ __launch_bounds__(256) void mv_sh_bwd_kernel(MVArgs a) {
   const float* pr = a.params;
   auto ch = [&](int c) { return pr[(size_t)c * P + g]; };
   const float3 p = make_float3(ch(DMGS_MV_XYZ), ch(DMGS_MV_XYZ + 1), ch(DMGS_MV_XYZ + 2));
-  float sh[48], dsh[48];
+  // the coefficients wait in LDS (column threadIdx.x: conflict-free) rather
+  // than in 48 registers, which leaves room for a third wave per SIMD
+  __shared__ float s_sh[48][256];
+  float dsh[48];
 #pragma unroll
   for (int k = 0; k < 48; ++k) {
-    sh[k] = ch(DMGS_MV_SH + k);
+    s_sh[k][threadIdx.x] = ch(DMGS_MV_SH + k);
     dsh[k] = 0.f;
   }
+  ShColumn sh{&s_sh[0][threadIdx.x]};
   float3 dmean = make_float3(0.f, 0.f, 0.f);
   auto col = [&](int v) {
     const float* gc = a.gacc + gacc_index(N, (size_t)v * P + g, DMGS_GACC_GEO);

@@ -278,7 +278,8 @@ __device__ __forceinline__ float3 ewa_backward(float3 p, const float c[6], const
 // the 16 basis values b (zero past the active degree) and the masked colour
 // gradient g, and returns dL/dmean through the normalised view direction.
 // The basis partials are those of the polynomials in sh_to_rgb.
-__device__ __forceinline__ float3 sh_backward(int deg, const float (&sh)[48], float3 pos, const float* campos,
+template <typename SH>
+__device__ __forceinline__ float3 sh_backward(int deg, const SH& sh, float3 pos, const float* campos,
                                               float3 dL_dRGB, float (&b)[16], float3& g) {
   const float3 v = make_float3(pos.x - campos[0], pos.y - campos[1], pos.z - campos[2]);
   const float inv_len = 1.0f / sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
