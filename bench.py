@@ -396,26 +396,35 @@ def _cpu_cam(cam):
 
 
 def _read_valu(kernel):
-    """Wave64 VALU instructions per launch of ``kernel`` at the default
-    workload (rocprofv3 SQ_INSTS_VALU, committed under profiles/), or None."""
+    """Counter figures of ``kernel`` at the default workload, committed under
+    profiles/: wave64 VALU instructions per launch (SQ_INSTS_VALU), VALU
+    busy quad-cycles per launch (SQ_ACTIVE_INST_VALU) and the launch's
+    duration in that counter run; or None."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "valu.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kernel)
+            v = json.load(f).get(kernel)
     except (OSError, ValueError):
         return None
+    return v if isinstance(v, dict) else None
 
 
 def _valu_issue(kernel, ms, args):
-    """The kernel's VALU issue rate against MI355X's (256 CUs x 4 SIMDs, one
-    wave64 instruction per SIMD per 4 cycles at 2.4 GHz), from its committed
-    instruction count and the live time; only for the default workload."""
-    n = _read_valu(kernel)
-    if n is None or args.workload != "views" or args.gaussians or args.views or (args.width, args.height) != (800, 800):
+    """VALU issue of ``kernel`` on MI355X (256 CUs x 4 SIMDs, one wave64
+    instruction per SIMD per 4 cycles at 2.4 GHz): the busy fraction from the
+    committed counter run (its busy cycles over its own duration, so both
+    come from one run), and the instruction rate at the live time (which
+    includes the stage's two small tile-order kernels); only for the default
+    workload."""
+    v = _read_valu(kernel)
+    if v is None or args.workload != "views" or args.gaussians or args.views or (args.width, args.height) != (800, 800):
         return None
-    peak = 256 * 4 * 2.4e9 / 4  # wave instructions per second
-    return {"instructions": n, "achieved_per_s": round(n / (ms * 1e-3), 1), "peak_per_s": peak,
-            "frac": round(n / (ms * 1e-3) / peak, 4)}
+    simds, clock = 256 * 4, 2.4e9
+    peak = simds * clock / 4  # wave instructions per second
+    n = v["instructions"]
+    return {"instructions": n, "profiled_us": v["profiled_us"],
+            "frac": round(v["active_quad_cycles"] * 4 / (simds * clock * v["profiled_us"] * 1e-6), 4),
+            "achieved_per_s_live": round(n / (ms * 1e-3), 1), "peak_per_s": peak}
 
 
 def _read_traffic(kernel):
