@@ -99,9 +99,8 @@ def test_preprocess_binning_forward(W, H, n):
                                 rotations=sc["rots"])
     ok2 = ~ost["ambiguous"].numpy()
     d2 = (color.cpu().double() - ocol).abs().numpy().max(0)
-    frac = (d2[ok2] < IMG_TOL).mean()
-    print(f"  end-to-end: max |diff| {d2[ok2].max():.3e}, within tol {frac:.5f}")
-    assert frac > 0.999
+    print(f"  end-to-end: max |diff| {d2[ok2].max():.3e} on {int(ok2.sum())} untied pixels")
+    assert d2[ok2].max() < IMG_TOL
 
 
 @pytest.mark.parametrize("mode", ["sh_scale_rot", "colors_cov"])
