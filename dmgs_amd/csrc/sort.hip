@@ -169,14 +169,26 @@ void launch_inclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t
 // ---------------------------------------------------------------------------
 // Lanes of the wave whose digit equals this lane's: one ballot per digit bit
 // (at most 8; the loop is unrolled with a uniform test of the width).
+// Per bit: s = the lane's bit sign-extended (all ones or zero); a lane j is
+// still a peer when its bit equals ours, i.e. where ballot ^ s is clear.
+__device__ __forceinline__ uint64_t digit_peers_bit(uint64_t peers, uint32_t d, int b) {
+  const uint32_t s = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+  const uint64_t bal = __ballot(s != 0u);
+  const uint32_t lo = (uint32_t)peers & ~((uint32_t)bal ^ s);
+  const uint32_t hi = (uint32_t)(peers >> 32) & ~((uint32_t)(bal >> 32) ^ s);
+  return ((uint64_t)hi << 32) | lo;
+}
 __device__ __forceinline__ uint64_t digit_peers(uint32_t d, int nbits, bool valid) {
   uint64_t peers = __ballot(valid);
+  if (nbits == 8) {  // every depth pass and the one-pass cell sort: no per-bit branch
+#pragma unroll
+    for (int b = 0; b < 8; ++b) peers = digit_peers_bit(peers, d, b);
+    return peers;
+  }
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
     if (b >= nbits) break;
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bal = __ballot(bit);
-    peers &= bit ? bal : ~bal;
+    peers = digit_peers_bit(peers, d, b);
   }
   return peers;
 }
