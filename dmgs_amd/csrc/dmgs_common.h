@@ -102,15 +102,13 @@ __device__ __forceinline__ int cell_count(int2 rmin, int2 rmax) {
 // the covered columns and rows of the cell, as an outer product of bit runs.
 __device__ __forceinline__ uint32_t cell_mask(int2 rmin, int2 rmax, int cx, int cy) {
   constexpr int E = 1 << DMGS_CELL_SHIFT;
+  static_assert(E == 4, "the row replication below is for cells of 4 x 4 tiles");
   const int x0 = max(rmin.x - cx * E, 0), x1 = min(rmax.x - cx * E, E);
   const int y0 = max(rmin.y - cy * E, 0), y1 = min(rmax.y - cy * E, E);
   if (x1 <= x0 || y1 <= y0) return 0u;
-  const uint32_t row = ((1u << (x1 - x0)) - 1u) << x0;  // columns of one cell row
-  uint32_t m = 0;
-#pragma unroll
-  for (int r = 0; r < E; ++r)
-    if (r >= y0 && r < y1) m |= row << (r * E);
-  return m;
+  const uint32_t row = ((1u << (x1 - x0)) - 1u) << x0;              // columns of one cell row
+  const uint32_t rows = ((1u << (E * (y1 - y0))) - 1u) << (E * y0);  // the covered rows' bits
+  return (row * 0x1111u) & rows;                                    // the row in all four rows, masked
 }
 
 // Tile rectangle [rmin, rmax) packed in 8 bytes: (x0 | x1 << 16, y0 | y1 << 16).
