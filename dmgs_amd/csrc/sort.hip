@@ -609,18 +609,33 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
   __syncthreads();
   uint32_t first = a.chunk_first[blockIdx.x];
   uint32_t win_begin = (first == 0) ? 0u : min(a.offsets[first - 1], K);
+  // A window's per-entry loads (entry, its slot range, its rectangle), issued
+  // together; the next window's are in flight while this one is emitted.
+  struct Fetched {
+    uint32_t g, prev_end, end;
+    uint2 pr;
+  };
+  auto fetch = [&](uint32_t s) {
+    Fetched f{0u, 0u, 0u, make_uint2(0u, 0u)};
+    if (s < P) {
+      f.g = a.order[s];
+      f.prev_end = a.offsets[s > 0 ? s - 1 : 0];  // unconditional: no branch between the loads
+      f.end = a.offsets[s];
+      if (a.views) f.pr = a.rects[s];
+    }
+    return f;
+  };
+  Fetched cur = fetch(first + threadIdx.x);
   while (true) {
-    // stage Gaussians first .. first+255 (those past P get an empty range);
-    // the loads of one entry are independent and issued together
+    // stage Gaussians first .. first+255 (those past P get an empty range)
     const uint32_t s = first + threadIdx.x;
     uint32_t end = 0xFFFFFFFFu, g = 0, tb = 0, rx = 0, ry = 0;
     int x0 = 0, y0 = 0, w = 1, gx = a.grid_x;
     if (s < P) {
-      g = a.order[s];
-      const uint32_t prev_end = a.offsets[s > 0 ? s - 1 : 0];  // unconditional: no branch between the loads
-      end = min(a.offsets[s], K);
-      const uint2 pr = a.views ? a.rects[s] : make_uint2(0u, 0u);
-      const uint32_t beg = (s == 0) ? 0u : min(prev_end, K);
+      g = cur.g;
+      end = min(cur.end, K);
+      const uint2 pr = cur.pr;
+      const uint32_t beg = (s == 0) ? 0u : min(cur.prev_end, K);
       if (end > beg && g < P) {
         int2 rmin, rmax;
         if (a.views) {  // depth-ordered rectangle from the gather step; per binning cell of 4x4 tiles
@@ -660,6 +675,8 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
     __syncthreads();
     const uint32_t win_end = min(s_end[255], hi_slot);
     const uint32_t o_begin = max(lo_slot, win_begin);
+    const bool more = win_end < hi_slot && first + 256 < P;  // uniform: another window follows
+    if (more) cur = fetch(first + 256 + threadIdx.x);
     // Four consecutive slots per thread: one binary search per group, then
     // the slots walk the entry's cells in order (a new entry starts at its
     // first cell), and a full group leaves as one 16-byte store per array.
@@ -720,7 +737,6 @@ __global__ __launch_bounds__(256) void duplicate_kernel(DupArgs a) {
         }
       }
     }
-    const bool more = win_end < hi_slot && first + 256 < P;
     win_begin = s_end[255];  // the next window starts where this one ends
     __syncthreads();
     if (!more) break;
