@@ -97,6 +97,19 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
   const int c = blockIdx.z;
   const int ox = blockIdx.x * kTW, oy = blockIdx.y * kTH;
   const size_t plane = (size_t)H * W;
+  // the target's maps at this thread's four outputs (computed once by
+  // dssim_target_kernel), requested together with the patches
+  float ys[4][2];
+  if constexpr (STATS) {
+    const int lx = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
+    const size_t ally = (size_t)gridDim.z * plane;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const size_t o = c * plane + (size_t)min(oy + r0 + i, H - 1) * W + min(ox + lx, W - 1);
+      ys[i][0] = ystats[o];
+      ys[i][1] = ystats[ally + o];
+    }
+  }
   {
     float(*const dst[2])[kPS] = {sx, sy};
     const float* const src[2] = {x + c * plane, y + c * plane};
@@ -159,14 +172,11 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
     }
   }
   const int px = ox + lx;
-  if constexpr (STATS) {  // the target's maps, computed once by dssim_target_kernel
-    const size_t ally = (size_t)gridDim.z * plane;
+  if constexpr (STATS) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const int py = oy + r0 + i;
-      const size_t o = c * plane + (size_t)min(py, H - 1) * W + min(px, W - 1);
-      v[i][3] = ystats[o];
-      v[i][4] = ystats[ally + o];
+      v[i][3] = ys[i][0];
+      v[i][4] = ys[i][1];
     }
   }
   const float C1 = 0.01f * 0.01f, C2 = 0.03f * 0.03f;
