@@ -122,12 +122,24 @@ class BatchedRenderer:
         # view-major lists (one cell-sort pass for views of up to 256 cells) where they apply
         fits = self.B <= 16 and self.max_view_cells <= 4096
         self.cell_mode = (1 if fits else 0) if cell_mode is None else int(cell_mode)
-        tv = torch.empty(self.total_tiles, dtype=torch.int32)
+        # both tables go through pinned host memory and asynchronous copies on
+        # the current stream: building a renderer never waits for the device
+        # (a render server builds one per batch while other batches run);
+        # the host buffers live as long as the renderer
+        pin = self.device.type == "cuda"
+        tv = torch.empty(self.total_tiles, dtype=torch.int32, pin_memory=pin)
         N.check(self.lib.dmgs_batched_tile_view(self.views, self.B, ctypes.c_void_p(tv.data_ptr()),
                                                 self.total_tiles), "dmgs_batched_tile_view")
-        self.tile_view = tv.to(self.device)
-        raw = bytes(memoryview(self.views).cast("B"))
-        self.views_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(self.device)
+        nbytes = ctypes.sizeof(self.views)
+        vh = torch.empty(nbytes, dtype=torch.uint8, pin_memory=pin)
+        ctypes.memmove(vh.data_ptr(), ctypes.addressof(self.views), nbytes)
+        self._host_tables = (tv, vh)
+        self.tile_view = tv.to(self.device, non_blocking=pin)
+        self.views_dev = vh.to(self.device, non_blocking=pin)
+        self._tables_ready = None
+        if pin:  # launches on another stream wait for the copies (see _after_tables)
+            self._tables_ready = torch.cuda.Event()
+            self._tables_ready.record()
         self.sizes = [(v.height, v.width) for v in self.views]
         self.pixel_base = [v.pixel_base for v in self.views]
         self.K_cap = 0
@@ -146,7 +158,15 @@ class BatchedRenderer:
         self.state = torch.empty(n, dtype=torch.uint8, device=self.device)
         self._offsets = list(offs)
 
+    def _after_tables(self):
+        """Order the current stream after the tables' copies (a no-op on the
+        stream that built the renderer; skipped under graph capture, which
+        starts from a synchronised device)."""
+        if self._tables_ready is not None and not torch.cuda.is_current_stream_capturing():
+            torch.cuda.current_stream(self.device).wait_event(self._tables_ready)
+
     def _desc(self, capacity=None):
+        self._after_tables()
         d = N.DmgsBatch()
         d.views, d.tile_view = self.views_dev.data_ptr(), self.tile_view.data_ptr()
         d.num_views, d.sh_degree, d.num_gaussians = self.B, self.sh_degree, self.P
