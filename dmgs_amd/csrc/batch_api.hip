@@ -386,6 +386,8 @@ int dmgs_batched_forward(const dmgs_batch* b, const float* params, float* images
     if (b->bwd_ws) {  // the blend work-groups also clear the backward accumulator
       r.clear = static_cast<float4*>(b->bwd_ws);
       r.clear_n = (c.N * DMGS_GACC_STRIDE + 3) / 4;  // inside the 256-byte-aligned workspace
+      r.clear_words = at<uint32_t>(b->state, c.S.bucket_hist);  // for the backward's tile order
+      r.clear_words_n = 512;
     }
     dmgs::launch_render_fwd(r, st, b->variant);
     if ((rc = bcheck("batch render forward", st))) return rc;
@@ -410,8 +412,9 @@ int dmgs_batched_backward(const dmgs_batch* b, const float* params, const float*
     dmgs::RenderArgs r = render_args(b, c);
     r.dL_dpix = dL_dimages;
     // re-order the tiles by the list lengths the strip forward measured
+    // (the forward blend, which every backward follows, zeroed the histogram)
     dmgs::launch_tile_order_bwd(r, at<uint32_t>(b->state, c.S.tile_order), at<uint8_t>(b->state, c.S.tile_bucket),
-                                at<uint32_t>(b->state, c.S.bucket_hist), st);
+                                at<uint32_t>(b->state, c.S.bucket_hist), st, true);
     dmgs::launch_render_bwd(r, st, b->variant);
     if ((rc = bcheck("batch render backward", st))) return rc;
   }

@@ -184,6 +184,10 @@ struct RenderArgs {
   // work-groups in equal shares after their blend work
   float4* clear;
   size_t clear_n;  // float4 count
+  // and these words (work-group 0): the tile-order histogram the backward's
+  // dispatch order starts from
+  uint32_t* clear_words;
+  int clear_words_n;
 };
 // Work-group shape / LDS batch depth of the blend kernels (results are the
 // same for every variant; the autotuner picks the fastest).  The strip
@@ -207,7 +211,8 @@ void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, ui
                        bool hist_zeroed = false);
 // The same with a.tile_work (written by the strip forward) as each tile's
 // cost: the backward's exact list length.
-void launch_tile_order_bwd(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s);
+void launch_tile_order_bwd(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s,
+                           bool hist_zeroed = false);
 void launch_render_bwd(const RenderArgs& a, hipStream_t s, int variant = kRender16x256);
 
 // ---- geometry backward ----

@@ -353,6 +353,8 @@ __device__ __forceinline__ TileCtx tile_ctx(const RenderArgs& a) {
 // A forward work-group's share of clearing the backward accumulator: run
 // after its blend work, so the stores delay no load of the blend.
 __device__ __forceinline__ void clear_share(const RenderArgs& a) {
+  if (a.clear_words && blockIdx.x == 0)
+    for (int i = threadIdx.x; i < a.clear_words_n; i += blockDim.x) a.clear_words[i] = 0u;
   if (!a.clear) return;
   const size_t per = (a.clear_n + gridDim.x - 1) / gridDim.x;
   const size_t lo = (size_t)blockIdx.x * per, hi = min(a.clear_n, lo + per);
@@ -1019,8 +1021,9 @@ void launch_tile_order(const RenderArgs& a, uint32_t* order, uint8_t* bucket, ui
   tile_order_impl(a, order, bucket, hist, false, s, hist_zeroed);
 }
 
-void launch_tile_order_bwd(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s) {
-  if (a.tile_work) tile_order_impl(a, order, bucket, hist, true, s, false);
+void launch_tile_order_bwd(const RenderArgs& a, uint32_t* order, uint8_t* bucket, uint32_t* hist, hipStream_t s,
+                           bool hist_zeroed) {
+  if (a.tile_work) tile_order_impl(a, order, bucket, hist, true, s, hist_zeroed);
 }
 
 void launch_render_fwd(const RenderArgs& a, hipStream_t s, int variant) {
