@@ -288,37 +288,68 @@ __device__ __forceinline__ float3 sh_backward(int deg, const SH& sh, float3 pos,
   for (int k = 0; k < 16; ++k) b[k] = 0.f;
   float3 rgb = make_float3(0.5f, 0.5f, 0.5f);
   float3 dRx = make_float3(0.f, 0.f, 0.f), dRy = dRx, dRz = dRx;
-  auto acc = [&](int k, float bk, float bxk, float byk, float bzk) {
-    const float s0 = sh[3 * k], s1 = sh[3 * k + 1], s2 = sh[3 * k + 2];
+  // basis value k, and each nonzero partial of it along x, y, z; the sums
+  // keep the order over k (a partial that is identically zero adds nothing)
+  auto basis = [&](int k, float bk) {
     b[k] = bk;
-    rgb.x += bk * s0; rgb.y += bk * s1; rgb.z += bk * s2;
-    dRx.x += bxk * s0; dRx.y += bxk * s1; dRx.z += bxk * s2;
-    dRy.x += byk * s0; dRy.y += byk * s1; dRy.z += byk * s2;
-    dRz.x += bzk * s0; dRz.y += bzk * s1; dRz.z += bzk * s2;
+    rgb.x += bk * sh[3 * k]; rgb.y += bk * sh[3 * k + 1]; rgb.z += bk * sh[3 * k + 2];
   };
-  acc(0, kSH_C0, 0.f, 0.f, 0.f);
+  auto part = [&](float3& dR, int k, float d) {
+    dR.x += d * sh[3 * k]; dR.y += d * sh[3 * k + 1]; dR.z += d * sh[3 * k + 2];
+  };
+  basis(0, kSH_C0);
   if (deg > 0) {
-    acc(1, -kSH_C1 * y, 0.f, -kSH_C1, 0.f);
-    acc(2, kSH_C1 * z, 0.f, 0.f, kSH_C1);
-    acc(3, -kSH_C1 * x, -kSH_C1, 0.f, 0.f);
+    basis(1, -kSH_C1 * y);
+    part(dRy, 1, -kSH_C1);
+    basis(2, kSH_C1 * z);
+    part(dRz, 2, kSH_C1);
+    basis(3, -kSH_C1 * x);
+    part(dRx, 3, -kSH_C1);
     if (deg > 1) {
       const float xx = x * x, yy = y * y, zz = z * z, xy = x * y, yz = y * z, xz = x * z;
-      acc(4, kSH_C2[0] * xy, kSH_C2[0] * y, kSH_C2[0] * x, 0.f);
-      acc(5, kSH_C2[1] * yz, 0.f, kSH_C2[1] * z, kSH_C2[1] * y);
-      acc(6, kSH_C2[2] * (2.f * zz - xx - yy), -2.f * kSH_C2[2] * x, -2.f * kSH_C2[2] * y, 4.f * kSH_C2[2] * z);
-      acc(7, kSH_C2[3] * xz, kSH_C2[3] * z, 0.f, kSH_C2[3] * x);
-      acc(8, kSH_C2[4] * (xx - yy), 2.f * kSH_C2[4] * x, -2.f * kSH_C2[4] * y, 0.f);
+      basis(4, kSH_C2[0] * xy);
+      part(dRx, 4, kSH_C2[0] * y);
+      part(dRy, 4, kSH_C2[0] * x);
+      basis(5, kSH_C2[1] * yz);
+      part(dRy, 5, kSH_C2[1] * z);
+      part(dRz, 5, kSH_C2[1] * y);
+      basis(6, kSH_C2[2] * (2.f * zz - xx - yy));
+      part(dRx, 6, -2.f * kSH_C2[2] * x);
+      part(dRy, 6, -2.f * kSH_C2[2] * y);
+      part(dRz, 6, 4.f * kSH_C2[2] * z);
+      basis(7, kSH_C2[3] * xz);
+      part(dRx, 7, kSH_C2[3] * z);
+      part(dRz, 7, kSH_C2[3] * x);
+      basis(8, kSH_C2[4] * (xx - yy));
+      part(dRx, 8, 2.f * kSH_C2[4] * x);
+      part(dRy, 8, -2.f * kSH_C2[4] * y);
       if (deg > 2) {
-        acc(9, kSH_C3[0] * y * (3.f * xx - yy), kSH_C3[0] * 6.f * xy, kSH_C3[0] * 3.f * (xx - yy), 0.f);
-        acc(10, kSH_C3[1] * xy * z, kSH_C3[1] * yz, kSH_C3[1] * xz, kSH_C3[1] * xy);
-        acc(11, kSH_C3[2] * y * (4.f * zz - xx - yy), kSH_C3[2] * (-2.f * xy), kSH_C3[2] * (4.f * zz - xx - 3.f * yy),
-            kSH_C3[2] * 8.f * yz);
-        acc(12, kSH_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy), kSH_C3[3] * (-6.f * xz), kSH_C3[3] * (-6.f * yz),
-            kSH_C3[3] * 3.f * (2.f * zz - xx - yy));
-        acc(13, kSH_C3[4] * x * (4.f * zz - xx - yy), kSH_C3[4] * (4.f * zz - 3.f * xx - yy), kSH_C3[4] * (-2.f * xy),
-            kSH_C3[4] * 8.f * xz);
-        acc(14, kSH_C3[5] * z * (xx - yy), kSH_C3[5] * 2.f * xz, kSH_C3[5] * (-2.f * yz), kSH_C3[5] * (xx - yy));
-        acc(15, kSH_C3[6] * x * (xx - 3.f * yy), kSH_C3[6] * 3.f * (xx - yy), kSH_C3[6] * (-6.f * xy), 0.f);
+        basis(9, kSH_C3[0] * y * (3.f * xx - yy));
+        part(dRx, 9, kSH_C3[0] * 6.f * xy);
+        part(dRy, 9, kSH_C3[0] * 3.f * (xx - yy));
+        basis(10, kSH_C3[1] * xy * z);
+        part(dRx, 10, kSH_C3[1] * yz);
+        part(dRy, 10, kSH_C3[1] * xz);
+        part(dRz, 10, kSH_C3[1] * xy);
+        basis(11, kSH_C3[2] * y * (4.f * zz - xx - yy));
+        part(dRx, 11, kSH_C3[2] * (-2.f * xy));
+        part(dRy, 11, kSH_C3[2] * (4.f * zz - xx - 3.f * yy));
+        part(dRz, 11, kSH_C3[2] * 8.f * yz);
+        basis(12, kSH_C3[3] * z * (2.f * zz - 3.f * xx - 3.f * yy));
+        part(dRx, 12, kSH_C3[3] * (-6.f * xz));
+        part(dRy, 12, kSH_C3[3] * (-6.f * yz));
+        part(dRz, 12, kSH_C3[3] * 3.f * (2.f * zz - xx - yy));
+        basis(13, kSH_C3[4] * x * (4.f * zz - xx - yy));
+        part(dRx, 13, kSH_C3[4] * (4.f * zz - 3.f * xx - yy));
+        part(dRy, 13, kSH_C3[4] * (-2.f * xy));
+        part(dRz, 13, kSH_C3[4] * 8.f * xz);
+        basis(14, kSH_C3[5] * z * (xx - yy));
+        part(dRx, 14, kSH_C3[5] * 2.f * xz);
+        part(dRy, 14, kSH_C3[5] * (-2.f * yz));
+        part(dRz, 14, kSH_C3[5] * (xx - yy));
+        basis(15, kSH_C3[6] * x * (xx - 3.f * yy));
+        part(dRx, 15, kSH_C3[6] * 3.f * (xx - yy));
+        part(dRy, 15, kSH_C3[6] * (-6.f * xy));
       }
     }
   }
