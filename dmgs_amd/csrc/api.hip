@@ -529,7 +529,7 @@ size_t dmgs_dssim_l1_workspace_size(int32_t C, int32_t H, int32_t W) {
 
 int dmgs_dssim_target_stats(const float* gt, int32_t C, int32_t H, int32_t W, float* stats, dmgs_stream_t stream) {
   if (!gt || !stats) return fail(DMGS_EINVAL, "null pointer");
-  if (C <= 0 || H <= 0 || W <= 0 || (int64_t)C * H * W > ((int64_t)1 << 31))
+  if (C <= 0 || H <= 0 || W <= 0 || H >= (1 << 24) || W >= (1 << 24) || (int64_t)C * H * W > ((int64_t)1 << 31))
     return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d", C, H, W);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   float w[11];
@@ -542,7 +542,8 @@ int dmgs_dssim_l1_forward_groups_stats(const float* img, const float* gt, const 
                                        int32_t Cg, int32_t H, int32_t W, float lambda_dssim, void* ws,
                                        float* loss_out, dmgs_stream_t stream) {
   if (!img || !gt || !ws || !loss_out) return fail(DMGS_EINVAL, "null pointer");
-  if (G <= 0 || Cg <= 0 || H <= 0 || W <= 0 || (int64_t)G * Cg * H * W > ((int64_t)1 << 31))
+  if (G <= 0 || Cg <= 0 || H <= 0 || W <= 0 || H >= (1 << 24) || W >= (1 << 24) ||
+      (int64_t)G * Cg * H * W > ((int64_t)1 << 31))
     return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d x %d", G, Cg, H, W);
   if (misaligned(ws)) return fail(DMGS_EINVAL, "ws must be 256-byte aligned");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -564,7 +565,8 @@ int dmgs_dssim_l1_backward_groups(const float* img, const float* gt, int32_t G, 
                                   float lambda_dssim, const void* ws, const float* dloss, float* dimg,
                                   dmgs_stream_t stream) {
   if (!img || !gt || !ws || !dloss || !dimg) return fail(DMGS_EINVAL, "null pointer");
-  if (G <= 0 || Cg <= 0 || H <= 0 || W <= 0 || (int64_t)G * Cg * H * W > ((int64_t)1 << 31))
+  if (G <= 0 || Cg <= 0 || H <= 0 || W <= 0 || H >= (1 << 24) || W >= (1 << 24) ||
+      (int64_t)G * Cg * H * W > ((int64_t)1 << 31))
     return fail(DMGS_EINVAL, "invalid image shape %d x %d x %d x %d", G, Cg, H, W);
   if (misaligned(ws)) return fail(DMGS_EINVAL, "ws must be 256-byte aligned");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);

@@ -43,31 +43,70 @@ struct Window11 {
 };
 
 // Stage zero-padded kPH x kPW patches of NP planes (origin at (oy - 5, ox - 5)).
-// Every global load of the thread is issued before the first LDS store, so
-// the workgroup waits for one memory latency, not one per element.
-constexpr int kStageIters = (kPH * kPW + 255) / 256;  // 7
+// Thread t < 252 owns patch column t % 42 and rows t / 42 + 6k (k = 0..6).
+// Every element is loaded from its coordinates clamped into the plane, so no
+// load is conditional, and replaced by zero outside the image; all loads of
+// the thread are issued before the first LDS store, so the workgroup waits
+// for one memory latency, not one per element.  Offsets within a plane are
+// 32-bit products of 24-bit factors (sides below 2^24, planes of at most
+// 2^31 pixels).
+constexpr int kStageGroups = 6;                    // row groups
+constexpr int kStageIters = kPH / kStageGroups;    // 7
+static_assert(kPH % kStageGroups == 0 && kPW * kStageGroups <= 256, "one patch column per thread and group");
 template <int NP>
 __device__ __forceinline__ void stage_patches(float (*const dst[NP])[kPS], const float* const src[NP], int H, int W,
                                               int ox, int oy) {
+  const int t = threadIdx.x;
+  if (t >= kPW * kStageGroups) return;
+  const int q = t % kPW, r0 = t / kPW;
+  const int gx = ox + q - kR;
+  const uint32_t cx = (uint32_t)min(max(gx, 0), W - 1);
+  const bool in_x = (uint32_t)gx < (uint32_t)W;
   float v[NP][kStageIters];
 #pragma unroll
-  for (int it = 0; it < kStageIters; ++it) {
-    const int i = threadIdx.x + it * 256;
-    const int r = i / kPW, q = i - r * kPW;
-    const int gy = oy + r - kR, gx = ox + q - kR;
-    const bool in = i < kPH * kPW && gy >= 0 && gy < H && gx >= 0 && gx < W;
-    const size_t o = in ? (size_t)gy * W + gx : 0;
+  for (int k = 0; k < kStageIters; ++k) {
+    const int gy = oy + r0 + kStageGroups * k - kR;
+    const uint32_t o = __umul24((uint32_t)min(max(gy, 0), H - 1), (uint32_t)W) + cx;
 #pragma unroll
-    for (int m = 0; m < NP; ++m) v[m][it] = in ? src[m][o] : 0.f;
+    for (int m = 0; m < NP; ++m) v[m][k] = src[m][o];
   }
 #pragma unroll
-  for (int it = 0; it < kStageIters; ++it) {
-    const int i = threadIdx.x + it * 256;
-    if (i >= kPH * kPW) break;
-    const int r = i / kPW, q = i - r * kPW;
+  for (int k = 0; k < kStageIters; ++k) {
+    const int gy = oy + r0 + kStageGroups * k - kR;
+    const bool in = in_x && (uint32_t)gy < (uint32_t)H;
 #pragma unroll
-    for (int m = 0; m < NP; ++m) dst[m][r][q] = v[m][it];
+    for (int m = 0; m < NP; ++m) dst[m][r0 + kStageGroups * k][q] = in ? v[m][k] : 0.f;
   }
+}
+
+// Offset of pixel (px, py) in its plane (both inside the image).
+__device__ __forceinline__ uint32_t pix(int px, int py, int W) {
+  return __umul24((uint32_t)py, (uint32_t)W) + (uint32_t)px;
+}
+
+// The tile of this work-group.  Work-groups are dealt to the GPU's 8 XCDs
+// in turn (linear id mod 8) and each XCD has its own L2, so consecutive ids
+// would put neighbouring tiles, whose 5-pixel margins overlap, on different
+// L2s and every margin line would come from HBM once per tile.  Instead XCD
+// k takes one contiguous run of tiles in (plane, row, column) order: a
+// tile's left neighbour ran just before it on the same XCD and the tile
+// above it one row of tiles earlier, so most margin lines are L2 hits.  A
+// bijection of the grid onto itself for any grid size.
+struct Tile {
+  int bx, by, bz;
+};
+__device__ __forceinline__ Tile xcd_tile() {
+  constexpr uint32_t kXcds = 8;
+  const uint32_t gx = gridDim.x, gy = gridDim.y;
+  const uint32_t n = gx * gy * gridDim.z;
+  const uint32_t L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const uint32_t k = L % kXcds, j = L / kXcds;
+  const uint32_t T = k * (n / kXcds) + min(k, n % kXcds) + j;
+  Tile t;
+  t.bx = (int)(T % gx);
+  t.by = (int)((T / gx) % gy);
+  t.bz = (int)(T / (gx * gy));
+  return t;
 }
 
 // The 14 staged values of row r starting at column 4j (16-byte aligned).
@@ -94,20 +133,22 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
   __shared__ __align__(16) float sy[kPH][kPS];
   __shared__ __align__(16) float hq[NQ][kPH][kHS];
   __shared__ float red[2][4];
-  const int c = blockIdx.z;
-  const int ox = blockIdx.x * kTW, oy = blockIdx.y * kTH;
+  const Tile tl = xcd_tile();
+  const int c = tl.bz;
+  const int ox = tl.bx * kTW, oy = tl.by * kTH;
   const size_t plane = (size_t)H * W;
   // the target's maps at this thread's four outputs (computed once by
   // dssim_target_kernel), requested together with the patches
   float ys[4][2];
   if constexpr (STATS) {
     const int lx = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
-    const size_t ally = (size_t)gridDim.z * plane;
+    const float* y0 = ystats + c * plane;
+    const float* y1 = y0 + (size_t)gridDim.z * plane;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const size_t o = c * plane + (size_t)min(oy + r0 + i, H - 1) * W + min(ox + lx, W - 1);
-      ys[i][0] = ystats[o];
-      ys[i][1] = ystats[ally + o];
+      const uint32_t o = pix(min(ox + lx, W - 1), min(oy + r0 + i, H - 1), W);
+      ys[i][0] = y0[o];
+      ys[i][1] = y1[o];
     }
   }
   {
@@ -182,6 +223,9 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
   const float C1 = 0.01f * 0.01f, C2 = 0.03f * 0.03f;
   const float dS = -lam / (float)((size_t)group * plane);  // dL/dS for the mean over the group's pixels
   const size_t all = (size_t)gridDim.z * plane;
+  float* const c0 = coef + c * plane;
+  float* const c1 = c0 + all;
+  float* const c2 = c1 + all;
   float s_acc = 0.f, l_acc = 0.f;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -200,10 +244,10 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
     const float dmu1 = inv * (2.f * mu2 * (A2 - A1) - S * (2.f * mu1) * (B2 - B1));
     const float ds11 = -S * B1 * inv;
     const float ds12 = 2.f * A1 * inv;
-    const size_t o = c * plane + (size_t)py * W + px;
-    coef[o] = dS * dmu1;
-    coef[all + o] = dS * ds11;
-    coef[2 * all + o] = dS * ds12;
+    const uint32_t o = pix(px, py, W);
+    c0[o] = dS * dmu1;
+    c1[o] = dS * ds11;
+    c2[o] = dS * ds12;
   }
   // block sums of S and |x - y|
   const float s_sum = wave_sum(s_acc), l_sum = wave_sum(l_acc);
@@ -214,7 +258,7 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const size_t b = ((size_t)tl.bz * gridDim.y + tl.by) * gridDim.x + tl.bx;
     partials[2 * b] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
     partials[2 * b + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
   }
@@ -226,8 +270,9 @@ __global__ __launch_bounds__(256) void dssim_target_kernel(const float* __restri
                                                            float* __restrict__ stats) {
   __shared__ __align__(16) float sy[kPH][kPS];
   __shared__ __align__(16) float hq[2][kPH][kHS];
-  const int c = blockIdx.z;
-  const int ox = blockIdx.x * kTW, oy = blockIdx.y * kTH;
+  const Tile tl = xcd_tile();
+  const int c = tl.bz;
+  const int ox = tl.bx * kTW, oy = tl.by * kTH;
   const size_t plane = (size_t)H * W;
   {
     float(*const dst[1])[kPS] = {sy};
@@ -275,14 +320,15 @@ __global__ __launch_bounds__(256) void dssim_target_kernel(const float* __restri
   }
   const int px = ox + lx;
   if (px >= W) return;
-  const size_t all = (size_t)gridDim.z * plane;
+  float* const s0 = stats + c * plane;
+  float* const s1 = s0 + (size_t)gridDim.z * plane;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int py = oy + r0 + i;
     if (py >= H) break;
-    const size_t o = c * plane + (size_t)py * W + px;
-    stats[o] = v[i][0];
-    stats[all + o] = v[i][1];
+    const uint32_t o = pix(px, py, W);
+    s0[o] = v[i][0];
+    s1[o] = v[i][1];
   }
 }
 
@@ -322,22 +368,26 @@ __global__ __launch_bounds__(256) void dssim_bwd_kernel(const float* __restrict_
                                                         const float* __restrict__ dloss, float* __restrict__ dx) {
   __shared__ __align__(16) float sa[3][kPH][kPS];
   __shared__ __align__(16) float hq[3][kPH][kHS];
-  const int c = blockIdx.z;
-  const int ox = blockIdx.x * kTW, oy = blockIdx.y * kTH;
+  const Tile tl = xcd_tile();
+  const int c = tl.bz;
+  const int ox = tl.bx * kTW, oy = tl.by * kTH;
   const size_t plane = (size_t)H * W;
   const size_t all = (size_t)C * plane;
   const int lx = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
   const int px = ox + lx;
   // the thread's own pixels of x and y, loaded while the patches stage
   float xv[4], yv[4];
+  {
+    const float* const xc = x + c * plane;
+    const float* const yc = y + c * plane;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int py = oy + r0 + i;
-    const bool in = px < W && py < H;
-    const size_t o = c * plane + (in ? (size_t)py * W + px : 0);
-    xv[i] = in ? x[o] : 0.f;
-    yv[i] = in ? y[o] : 0.f;
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t o = pix(min(px, W - 1), min(oy + r0 + i, H - 1), W);
+      xv[i] = xc[o];
+      yv[i] = yc[o];
+    }
   }
+  const float scale = dloss[c / group];
   {
     float(*const dst[3])[kPS] = {sa[0], sa[1], sa[2]};
     const float* const src[3] = {coef + c * plane, coef + all + c * plane, coef + 2 * all + c * plane};
@@ -379,16 +429,16 @@ __global__ __launch_bounds__(256) void dssim_bwd_kernel(const float* __restrict_
       }
     }
   if (px >= W) return;
-  const float scale = dloss[c / group];
   const float g_l1 = (1.f - lam) / (float)((size_t)group * plane);
+  float* const dxc = dx + c * plane;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int py = oy + r0 + i;
     if (py >= H) break;
-    const size_t o = c * plane + (size_t)py * W + px;
+    const uint32_t o = pix(px, py, W);
     const float d = xv[i] - yv[i];
     const float sgn = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
-    dx[o] = scale * (b[i][0] + 2.f * xv[i] * b[i][1] + yv[i] * b[i][2] + g_l1 * sgn);
+    dxc[o] = scale * (b[i][0] + 2.f * xv[i] * b[i][1] + yv[i] * b[i][2] + g_l1 * sgn);
   }
 }
 

@@ -206,7 +206,8 @@ int dmgs_faces_in_frustum(int64_t V, const float* verts, int64_t F, const int32_
  * sigma 1.5, zero padding, C1 = 1e-4, C2 = 9e-4, mean over C*H*W), objective as
  * train_geo_stage3.py:122-124.  Forward writes (loss, l1, ssim) to loss_out
  * (3 device floats) and keeps per-pixel coefficients in ws for the backward;
- * backward writes dimg = dloss[0] * d(loss)/d(img). */
+ * backward writes dimg = dloss[0] * d(loss)/d(img).  H and W below 2^24, at
+ * most 2^31 values per call (DMGS_EINVAL otherwise). */
 size_t dmgs_dssim_l1_workspace_size(int32_t C, int32_t H, int32_t W);
 int dmgs_dssim_l1_forward(const float* img, const float* gt, int32_t C, int32_t H, int32_t W, float lambda_dssim,
                           void* ws, float* loss_out, dmgs_stream_t stream);
