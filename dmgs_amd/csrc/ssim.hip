@@ -129,9 +129,12 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
                                                         Window11 g, float lam, float* __restrict__ coef,
                                                         float* __restrict__ partials) {
   constexpr int NQ = STATS ? 3 : 5;  // blurred maps: x, x^2, xy (, y, y^2)
-  __shared__ __align__(16) float sx[kPH][kPS];
-  __shared__ __align__(16) float sy[kPH][kPS];
-  __shared__ __align__(16) float hq[NQ][kPH][kHS];
+  // the x and y patches until every row blur has read them, then the
+  // row-blurred maps (one buffer: more work-groups per CU)
+  __shared__ __align__(16) union {
+    float sxy[2][kPH][kPS];
+    float hq[NQ][kPH][kHS];
+  } u;
   __shared__ float red[2][4];
   const Tile tl = xcd_tile();
   const int c = tl.bz;
@@ -152,19 +155,34 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
     }
   }
   {
-    float(*const dst[2])[kPS] = {sx, sy};
+    float(*const dst[2])[kPS] = {u.sxy[0], u.sxy[1]};
     const float* const src[2] = {x + c * plane, y + c * plane};
     stage_patches<2>(dst, src, H, W, ox, oy);
   }
   __syncthreads();
-  // horizontal: four outputs of one staged row per task
-  for (int t = threadIdx.x; t < kHTasks; t += 256) {
+  const int lx = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
+  const int px = ox + lx;
+  // |x - y| at the thread's four outputs, before the patches are overwritten
+  float l_acc = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    if (px < W && oy + r0 + i < H) l_acc += fabsf(u.sxy[0][r0 + i + kR][lx + kR] - u.sxy[1][r0 + i + kR][lx + kR]);
+  // horizontal: four outputs of one staged row per task, held in registers
+  // until every task has read the patches; map order x, x^2, xy (, y, y^2)
+  static_assert(kHTasks <= 2 * 256, "two row tasks per thread at most");
+  float res[2][NQ][4];
+#pragma unroll
+  for (int rep = 0; rep < 2; ++rep) {
+    const int t = threadIdx.x + rep * 256;
+    if (t >= kHTasks) break;
     const int r = t / kQuads, j = t - r * kQuads;
     float a[14], b[14];
-    load14(sx, r, j, a);
-    load14(sy, r, j, b);
-    float m1[4] = {0.f, 0.f, 0.f, 0.f}, m2[4] = {0.f, 0.f, 0.f, 0.f}, s11[4] = {0.f, 0.f, 0.f, 0.f},
-          s22[4] = {0.f, 0.f, 0.f, 0.f}, s12[4] = {0.f, 0.f, 0.f, 0.f};
+    load14(u.sxy[0], r, j, a);
+    load14(u.sxy[1], r, j, b);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) res[rep][q][i] = 0.f;
 #pragma unroll
     for (int e = 0; e < 14; ++e) {
       const float aa = a[e] * a[e], bb = b[e] * b[e], ab = a[e] * b[e];
@@ -173,27 +191,29 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
         const int k = e - i;
         if (k < 0 || k > 10) continue;
         const float w = g.w[k];
-        m1[i] = fmaf(w, a[e], m1[i]);
-        s11[i] = fmaf(w, aa, s11[i]);
-        s12[i] = fmaf(w, ab, s12[i]);
+        res[rep][0][i] = fmaf(w, a[e], res[rep][0][i]);
+        res[rep][1][i] = fmaf(w, aa, res[rep][1][i]);
+        res[rep][2][i] = fmaf(w, ab, res[rep][2][i]);
         if constexpr (!STATS) {
-          m2[i] = fmaf(w, b[e], m2[i]);
-          s22[i] = fmaf(w, bb, s22[i]);
+          res[rep][3][i] = fmaf(w, b[e], res[rep][3][i]);
+          res[rep][4][i] = fmaf(w, bb, res[rep][4][i]);
         }
       }
     }
-    // map order: x, x^2, xy, then y, y^2
-    *reinterpret_cast<float4*>(&hq[0][r][4 * j]) = make_float4(m1[0], m1[1], m1[2], m1[3]);
-    *reinterpret_cast<float4*>(&hq[1][r][4 * j]) = make_float4(s11[0], s11[1], s11[2], s11[3]);
-    *reinterpret_cast<float4*>(&hq[2][r][4 * j]) = make_float4(s12[0], s12[1], s12[2], s12[3]);
-    if constexpr (!STATS) {
-      *reinterpret_cast<float4*>(&hq[3][r][4 * j]) = make_float4(m2[0], m2[1], m2[2], m2[3]);
-      *reinterpret_cast<float4*>(&hq[4][r][4 * j]) = make_float4(s22[0], s22[1], s22[2], s22[3]);
-    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rep = 0; rep < 2; ++rep) {
+    const int t = threadIdx.x + rep * 256;
+    if (t >= kHTasks) break;
+    const int r = t / kQuads, j = t - r * kQuads;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      *reinterpret_cast<float4*>(&u.hq[q][r][4 * j]) =
+          make_float4(res[rep][q][0], res[rep][q][1], res[rep][q][2], res[rep][q][3]);
   }
   __syncthreads();
   // vertical: four outputs of one column per thread
-  const int lx = threadIdx.x & 31, r0 = (threadIdx.x >> 5) * 4;
   float v[4][5];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -203,7 +223,7 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
   for (int e = 0; e < 14; ++e) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const float h = hq[q][r0 + e][lx];
+      const float h = u.hq[q][r0 + e][lx];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int k = e - i;
@@ -212,7 +232,6 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
       }
     }
   }
-  const int px = ox + lx;
   if constexpr (STATS) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -226,7 +245,7 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
   float* const c0 = coef + c * plane;
   float* const c1 = c0 + all;
   float* const c2 = c1 + all;
-  float s_acc = 0.f, l_acc = 0.f;
+  float s_acc = 0.f;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int py = oy + r0 + i;
@@ -239,7 +258,6 @@ __global__ __launch_bounds__(256) void dssim_fwd_kernel(const float* __restrict_
     const float inv = 1.f / (B1 * B2);
     const float S = (A1 * A2) * inv;
     s_acc += S;
-    l_acc += fabsf(sx[r0 + i + kR][lx + kR] - sy[r0 + i + kR][lx + kR]);
     // 1/B1 = B2 inv and 1/B2 = B1 inv
     const float dmu1 = inv * (2.f * mu2 * (A2 - A1) - S * (2.f * mu1) * (B2 - B1));
     const float ds11 = -S * B1 * inv;
@@ -366,8 +384,12 @@ __global__ __launch_bounds__(256) void dssim_bwd_kernel(const float* __restrict_
                                                         const float* __restrict__ coef, int H, int W, int C,
                                                         int group, Window11 g, float lam,
                                                         const float* __restrict__ dloss, float* __restrict__ dx) {
-  __shared__ __align__(16) float sa[3][kPH][kPS];
-  __shared__ __align__(16) float hq[3][kPH][kHS];
+  // the coefficient patches until every row blur has read them, then the
+  // row-blurred coefficients (one buffer: more work-groups per CU)
+  __shared__ __align__(16) union {
+    float sa[3][kPH][kPS];
+    float hq[3][kPH][kHS];
+  } u;
   const Tile tl = xcd_tile();
   const int c = tl.bz;
   const int ox = tl.bx * kTW, oy = tl.by * kTH;
@@ -389,28 +411,46 @@ __global__ __launch_bounds__(256) void dssim_bwd_kernel(const float* __restrict_
   }
   const float scale = dloss[c / group];
   {
-    float(*const dst[3])[kPS] = {sa[0], sa[1], sa[2]};
+    float(*const dst[3])[kPS] = {u.sa[0], u.sa[1], u.sa[2]};
     const float* const src[3] = {coef + c * plane, coef + all + c * plane, coef + 2 * all + c * plane};
     stage_patches<3>(dst, src, H, W, ox, oy);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < kHTasks; t += 256) {
+  // horizontal: four outputs per task and map, held in registers until every
+  // task has read the patches
+  static_assert(kHTasks <= 2 * 256, "two row tasks per thread at most");
+  float res[2][3][4];
+#pragma unroll
+  for (int rep = 0; rep < 2; ++rep) {
+    const int t = threadIdx.x + rep * 256;
+    if (t >= kHTasks) break;
     const int r = t / kQuads, j = t - r * kQuads;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       float a[14];
-      load14(sa[m], r, j, a);
-      float h[4] = {0.f, 0.f, 0.f, 0.f};
+      load14(u.sa[m], r, j, a);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) res[rep][m][i] = 0.f;
 #pragma unroll
       for (int e = 0; e < 14; ++e)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int k = e - i;
           if (k < 0 || k > 10) continue;
-          h[i] = fmaf(g.w[k], a[e], h[i]);
+          res[rep][m][i] = fmaf(g.w[k], a[e], res[rep][m][i]);
         }
-      *reinterpret_cast<float4*>(&hq[m][r][4 * j]) = make_float4(h[0], h[1], h[2], h[3]);
     }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rep = 0; rep < 2; ++rep) {
+    const int t = threadIdx.x + rep * 256;
+    if (t >= kHTasks) break;
+    const int r = t / kQuads, j = t - r * kQuads;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      *reinterpret_cast<float4*>(&u.hq[m][r][4 * j]) =
+          make_float4(res[rep][m][0], res[rep][m][1], res[rep][m][2], res[rep][m][3]);
   }
   __syncthreads();
   float b[4][3];
@@ -420,7 +460,7 @@ __global__ __launch_bounds__(256) void dssim_bwd_kernel(const float* __restrict_
   for (int e = 0; e < 14; ++e)
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      const float h = hq[m][r0 + e][lx];
+      const float h = u.hq[m][r0 + e][lx];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int k = e - i;
