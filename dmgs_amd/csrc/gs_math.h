@@ -212,7 +212,10 @@ __device__ __forceinline__ float3 ewa_backward(float3 p, const float c[6], const
   const float ty = fminf(limy, fmaxf(-limy, tytz)) * t.z;
   const float xmul = (txtz < -limx || txtz > limx) ? 0.f : 1.f;
   const float ymul = (tytz < -limy || tytz > limy) ? 0.f : 1.f;
-  const float iz = 1.0f / t.z, iz2 = iz * iz, iz3 = iz2 * iz;
+  // Reciprocals that only scale gradients use the hardware rcp (1 ulp): t.z
+  // >= 0.2, denom^2 + 1e-7 and w > 0 are normal numbers.  The clamp tests
+  // above keep the forward's IEEE quotients, so both passes decide alike.
+  const float iz = __builtin_amdgcn_rcpf(t.z), iz2 = iz * iz, iz3 = iz2 * iz;
   const float fx = v.fx, fy = v.fy;
   const float j00 = fx * iz, j02 = -(fx * tx) * iz2, j11 = fy * iz, j12 = -(fy * ty) * iz2;
   const float V0[3] = {vm[0], vm[4], vm[8]}, V1[3] = {vm[1], vm[5], vm[9]}, V2[3] = {vm[2], vm[6], vm[10]};
@@ -234,7 +237,7 @@ __device__ __forceinline__ float3 ewa_backward(float3 p, const float c[6], const
   const float cc = m1[0] * S1[0] + m1[1] * S1[1] + m1[2] * S1[2] + 0.3f;
   // conic -> cov2D
   const float denom = ca * cc - cb * cb;
-  const float d2inv = 1.0f / ((denom * denom) + 0.0000001f);
+  const float d2inv = __builtin_amdgcn_rcpf((denom * denom) + 0.0000001f);
   float dLa = 0.f, dLb = 0.f, dLc = 0.f;
   if (d2inv != 0.f) {
     dLa = d2inv * (-cc * cc * dqa + 2.f * cb * cc * dqb + (denom - ca * cc) * dqc);
@@ -266,7 +269,7 @@ __device__ __forceinline__ float3 ewa_backward(float3 p, const float c[6], const
                              vm[8] * dtx + vm[9] * dty + vm[10] * dtz);
   // projection (means2D gradient is in NDC)
   const float4 ph = xform_point_4x4(p, pm);
-  const float mw = 1.0f / (ph.w + 0.0000001f);
+  const float mw = __builtin_amdgcn_rcpf(ph.w + 0.0000001f);
   const float mul1 = ph.x * mw * mw, mul2 = ph.y * mw * mw;
   dmean.x += (pm[0] * mw - pm[3] * mul1) * dmx + (pm[1] * mw - pm[3] * mul2) * dmy;
   dmean.y += (pm[4] * mw - pm[7] * mul1) * dmx + (pm[5] * mw - pm[7] * mul2) * dmy;
@@ -356,9 +359,10 @@ __device__ __forceinline__ float3 sh_backward(int deg, const SH& sh, float3 pos,
   g = make_float3(rgb.x < 0.f ? 0.f : dL_dRGB.x, rgb.y < 0.f ? 0.f : dL_dRGB.y, rgb.z < 0.f ? 0.f : dL_dRGB.z);
   const float3 ddir = make_float3(dRx.x * g.x + dRx.y * g.y + dRx.z * g.z, dRy.x * g.x + dRy.y * g.y + dRy.z * g.z,
                                   dRz.x * g.x + dRz.y * g.y + dRz.z * g.z);
-  // d normalize(v) / dv applied to ddir
+  // d normalize(v) / dv applied to ddir (|v|^-3 only scales the gradient:
+  // hardware reciprocal square root, 1 ulp)
   const float s2 = v.x * v.x + v.y * v.y + v.z * v.z;
-  const float inv32 = 1.0f / sqrtf(s2 * s2 * s2);
+  const float inv32 = __builtin_amdgcn_rsqf(s2 * s2 * s2);
   return make_float3(((s2 - v.x * v.x) * ddir.x - v.y * v.x * ddir.y - v.z * v.x * ddir.z) * inv32,
                      (-v.x * v.y * ddir.x + (s2 - v.y * v.y) * ddir.y - v.z * v.y * ddir.z) * inv32,
                      (-v.x * v.z * ddir.x - v.y * v.z * ddir.y + (s2 - v.z * v.z) * ddir.z) * inv32);
