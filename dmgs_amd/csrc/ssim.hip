@@ -7,8 +7,10 @@
 // the mean absolute difference.  The 2-D window is the outer product of the
 // normalised 1-D window, so both passes blur separably through LDS.
 //
-// Forward: one 32x32 output tile per workgroup (plane in blockIdx.z) stages
-// the 42x42 patch of both images in LDS.  The horizontal blur gives each
+// Forward: one 32x32 output tile per workgroup (the grid's x, y, plane;
+// xcd_tile maps work-group ids to tiles so that each XCD works through one
+// contiguous run of them) stages the 42x42 patch of both images in LDS.
+// The horizontal blur gives each
 // task four neighbouring outputs of one patch row from 14 loaded values
 // (x, y and the three products formed once per value); the vertical blur
 // gives each thread four outputs of one column from 14 blurred rows.  The
