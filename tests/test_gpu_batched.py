@@ -120,7 +120,7 @@ def _check_against_single_view(model, cams):
         err = (r.image(images, v) - imgs[v]).abs()[:, ok].max().item()
         print(f"view {v}: {int((~ok).sum())} tie pixels, max image error {err:.3g}", flush=True)
         assert err < IMG_TOL, f"view {v}: image error {err}"
-    bg = BT.unpack_grads(grads)
+    bg = BT.unpack_grads(grads, (model.max_sh_degree + 1) ** 2)
     for k in RAW:
         e = grad_rel_err(bg[k], pgrads[k])
         assert e < GRAD_TOL, f"{k}: rel err {e}"
@@ -154,6 +154,46 @@ def test_batched_empty_and_tiny_views(kinds):
     if "scene" not in kinds:
         for name in RAW:
             assert grads[name].abs().max().item() == 0.0, name
+
+
+def test_clamped_colour_channel_has_zero_sh_gradient():
+    """A colour channel that comes out of the SH negative is clamped to zero in
+    the forward and passes no gradient to its SH coefficients (the reference's
+    per-channel clamp flags).  With degree-0 SH a Gaussian's colour is the same
+    in every view, so Gaussians whose green channel is negative get exactly
+    zero green coefficient gradients, from the batched engine and from the
+    single-view path, while their red and blue channels still get gradient."""
+    print("case: clamped colour channel, 3000 Gaussians, 3 views, SH degree 0", flush=True)
+    model = S.free_scene(3000, seed=8, layout="mixture", shell_radius=3.0, sh_degree=0)
+    n_neg = 1500
+    model._features_dc[:n_neg, 0, 1] = -3.0  # green = C0 * (-3) + 0.5 < 0
+    cams = _cameras([(96, 64), (80, 80), (64, 96)], seed=12)
+    r, images, bgrads = _check_against_single_view(model, cams)
+    dc = bgrads["_features_dc"]
+    assert torch.count_nonzero(dc[:n_neg, 0, 1]).item() == 0
+    assert torch.count_nonzero(dc[:n_neg, 0, 0]).item() > n_neg // 10
+    assert torch.count_nonzero(dc[n_neg:, 0, 1]).item() > 0
+    g = torch.Generator().manual_seed(13)
+    dL_views = [torch.randn(3, c.image_height, c.image_width, generator=g).cuda() for c in cams]
+    _, pgrads, _ = _plain(model, cams, _bgs(len(cams)), dL_views)
+    assert torch.count_nonzero(pgrads["_features_dc"][:n_neg, 0, 1]).item() == 0
+    assert torch.count_nonzero(pgrads["_features_dc"][:n_neg, 0, 2]).item() > n_neg // 10
+
+
+def test_huge_splats_cover_every_cell():
+    """Three faint Gaussians of standard deviation 2.5 at the centre cover
+    every tile and cell of every view (hundreds of instances each, which the
+    load-balanced duplicate spreads over several chunks), among 3000 small
+    ones: the batched engine still matches the single-view path."""
+    print("case: huge splats, 3000 Gaussians, 3 views", flush=True)
+    model = S.free_scene(3000, seed=9, layout="mixture", shell_radius=3.0)
+    model._xyz[:3] = torch.tensor([[0.0, 0.0, 0.0], [0.3, -0.2, 0.1], [-0.2, 0.1, 0.3]])
+    model._scaling[:3] = math.log(2.5)
+    model._opacity[:3] = -2.5  # sigmoid: 0.076, so the splats behind them still show
+    cams = _cameras([(200, 150), (128, 96), (96, 160)], seed=14)
+    r, images, _ = _check_against_single_view(model, cams)
+    radii = r.radii().reshape(r.B, r.P)[:, :3]
+    assert (radii > 100).all(), radii
 
 
 def test_batched_matches_single_view_at_large_sort_size():
