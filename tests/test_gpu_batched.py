@@ -62,12 +62,12 @@ def _plain(model_cpu, cams, bgs, dL_views):
     return imgs, {k: getattr(m, k).grad for k in RAW}, counts
 
 
-def _ambiguous(model_cpu, cam, bg):
+def _ambiguous(model_cpu, cam, bg, sh_degree=3):
     """Pixels the FP64 oracle flags as blend threshold ties for this view
     (alpha ~ 1/255, T ~ 1e-4, power ~ 0): FP32 paths that differ by an ulp
     may resolve them differently, so they stay out of the comparison."""
     m = model_cpu
-    st = OR.settings_for(_cpu(cam), bg.double(), 3)
+    st = OR.settings_for(_cpu(cam), bg.double(), sh_degree)
     _, _, state = OR.rasterize(m.get_xyz, m.get_opacity, st, shs=m.get_features, scales=m.get_scaling,
                                rotations=m.get_rotation)
     return state["ambiguous"]
@@ -107,7 +107,7 @@ def _check_against_single_view(model, cams):
     bgs = _bgs(len(cams))
     r, params = _batched(model, cams, bgs)
     dL = _dL(r)
-    amb = [_ambiguous(model, c, b) for c, b in zip(cams, bgs)]
+    amb = [_ambiguous(model, c, b, sh_degree=model.max_sh_degree) for c, b in zip(cams, bgs)]
     for v in range(r.B):  # keep threshold ties out of the gradient comparison
         r.image(dL, v)[:, amb[v].to(r.device)] = 0.0
     images = r.forward(params)
@@ -178,6 +178,52 @@ def test_clamped_colour_channel_has_zero_sh_gradient():
     _, pgrads, _ = _plain(model, cams, _bgs(len(cams)), dL_views)
     assert torch.count_nonzero(pgrads["_features_dc"][:n_neg, 0, 1]).item() == 0
     assert torch.count_nonzero(pgrads["_features_dc"][:n_neg, 0, 2]).item() > n_neg // 10
+
+
+@pytest.mark.parametrize("deg", [1, 2])
+def test_batched_lower_sh_degrees(deg):
+    """Models of SH degree 1 and 2 (4 and 9 coefficients per channel, the rest
+    of the 48 packed channels unused): images and every raw-parameter
+    gradient match the single-view path."""
+    print(f"case: batched vs single-view, SH degree {deg}, 4000 Gaussians, 2 views", flush=True)
+    model = S.free_scene(4000, seed=10 + deg, layout="mixture", shell_radius=3.0, sh_degree=deg)
+    _, _, g = _check_against_single_view(model, _cameras([(112, 80), (72, 104)], seed=15))
+    assert g["_features_rest"].shape[1:] == ((deg + 1) ** 2 - 1, 3)
+    assert torch.count_nonzero(g["_features_rest"]).item() > 0
+
+
+def test_batched_active_degree_below_model_degree():
+    """The batched engine at active SH degree 1 on a degree-3 model (all 48
+    channels packed) equals the single-view path at the same active degree,
+    and the coefficients above degree 1 get exactly zero gradient."""
+    print("case: batched active SH degree 1 of 3, 4000 Gaussians, 2 views", flush=True)
+    model = S.free_scene(4000, seed=16, layout="mixture", shell_radius=3.0)
+    cams = _cameras([(112, 80), (72, 104)], seed=17)
+    bgs = _bgs(len(cams))
+    r = BT.BatchedRenderer(cams, [b.cuda() for b in bgs], model.num, sh_degree=1)
+    params = BT.pack_model(model.to("cuda"))
+    r.plan(params)
+    dL = _dL(r)
+    amb = [_ambiguous(model, c, b, sh_degree=1) for c, b in zip(cams, bgs)]
+    for v in range(r.B):
+        r.image(dL, v)[:, amb[v].to(r.device)] = 0.0
+    images = r.forward(params)
+    assert not r.overflowed()
+    grads = BT.unpack_grads(r.backward(params, dL))
+    m = model.to("cuda").requires_grad_(True)
+    m.active_sh_degree = 1
+    for v, (cam, bg) in enumerate(zip(cams, bgs)):
+        out = render(cam, m, PipelineParams(), bg.cuda())
+        (out["render"] * r.image(dL, v)).sum().backward()
+        ok = ~amb[v].to(r.device)
+        err = (r.image(images, v) - out["render"].detach()).abs()[:, ok].max().item()
+        print(f"view {v}: max image error {err:.3g}", flush=True)
+        assert err < IMG_TOL
+    for k in RAW:
+        e = grad_rel_err(grads[k], getattr(m, k).grad)
+        assert e < GRAD_TOL, f"{k}: rel err {e}"
+    assert torch.count_nonzero(grads["_features_rest"][:, 3:]).item() == 0
+    assert torch.count_nonzero(grads["_features_rest"][:, :3]).item() > 0
 
 
 def test_huge_splats_cover_every_cell():

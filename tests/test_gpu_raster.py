@@ -139,6 +139,46 @@ def test_backward_matches_oracle(mode):
     assert e < GRAD_TOL
 
 
+def test_active_sh_degree_below_model_degree():
+    """Training raises the active SH degree step by step while the features
+    keep all 16 coefficients: with settings.sh_degree = 1 on a degree-3 model
+    the colours use the first 4 coefficients only (image against the oracle
+    evaluated at degree 1), every gradient matches the oracle, and the
+    coefficients above the active degree get exactly zero gradient."""
+    print("case: active SH degree 1 of 3", flush=True)
+    W, H, n = 96, 80, 2000
+    sc = _scene(n, seed=5)
+    cam = _cam(W, H)
+    bg = torch.tensor([0.3, 0.1, 0.6])
+    rs = settings_from_cam(cam, bg, 1)
+    color, radii, state, _, _ = _run(sc, rs)
+    ors = oracle_settings(rs)
+    kw = dict(shs=sc["shs"], scales=sc["scales"], rotations=sc["rots"])
+    ocol, _, ost = OR.rasterize(sc["means3D"], sc["opac"], ors, **kw)
+    ok = ~ost["ambiguous"].numpy()
+    d = (color.cpu().double() - ocol).abs().numpy().max(0)
+    print(f"  image: max |diff| {d[ok].max():.3e} on {int(ok.sum())} untied pixels")
+    assert d[ok].max() < IMG_TOL
+    ocol3, _, _ = OR.rasterize(sc["means3D"], sc["opac"], oracle_settings(settings_from_cam(cam, bg, 3)), **kw)
+    assert (ocol3 - ocol).abs().max().item() > 1e-3  # degree 3 would give another image
+    g = torch.Generator().manual_seed(9)
+    dL = torch.randn(3, H, W, generator=g)
+    dL[:, ost["ambiguous"]] = 0.0
+    gg = RZ.backward_raw(state, radii, dL.cuda())
+    og = OR.rasterize_backward(ost, ors, sc["means3D"], dL.double(), **kw)
+    for i, nm in enumerate(["means3D", "means2D", "sh", "colors", "opacity", "scales", "rotations"]):
+        ref, got = og[i], gg[nm]
+        if ref is None:
+            continue
+        if nm == "means2D":
+            got, ref = got[:, :2], ref[:, :2]
+        e = grad_rel_err(got, ref)
+        print(f"  {nm}: rel err {e:.2e}")
+        assert e < GRAD_TOL, f"{nm} gradient rel err {e}"
+    assert torch.count_nonzero(gg["sh"][:, 4:]).item() == 0
+    assert torch.count_nonzero(gg["sh"][:, 1:4]).item() > 0
+
+
 def test_sort_pairs_stable():
     from dmgs_amd import _native as N
     rng = np.random.default_rng(0)
