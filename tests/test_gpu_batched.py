@@ -226,6 +226,42 @@ def test_batched_active_degree_below_model_degree():
     assert torch.count_nonzero(grads["_features_rest"][:, :3]).item() > 0
 
 
+def test_batched_scale_modifier():
+    """The batched engine with scale_modifier 0.6 equals the single-view
+    render(scaling_modifier=0.6): images off ties and every raw-parameter
+    gradient, including the log-scales'."""
+    print("case: batched scale modifier 0.6, 4000 Gaussians, 2 views", flush=True)
+    model = S.free_scene(4000, seed=18, layout="mixture", shell_radius=3.0)
+    cams = _cameras([(112, 80), (72, 104)], seed=19)
+    bgs = _bgs(len(cams))
+    r = BT.BatchedRenderer(cams, [b.cuda() for b in bgs], model.num, sh_degree=3, scale_modifier=0.6)
+    params = BT.pack_model(model.to("cuda"))
+    r.plan(params)
+    dL = _dL(r)
+    amb = []
+    for c, b in zip(cams, bgs):
+        st = OR.settings_for(_cpu(c), b.double(), 3, scale_modifier=0.6)
+        _, _, ost = OR.rasterize(model.get_xyz, model.get_opacity, st, shs=model.get_features,
+                                 scales=model.get_scaling, rotations=model.get_rotation)
+        amb.append(ost["ambiguous"])
+    for v in range(r.B):
+        r.image(dL, v)[:, amb[v].to(r.device)] = 0.0
+    images = r.forward(params)
+    assert not r.overflowed()
+    grads = BT.unpack_grads(r.backward(params, dL))
+    m = model.to("cuda").requires_grad_(True)
+    for v, (cam, bg) in enumerate(zip(cams, bgs)):
+        out = render(cam, m, PipelineParams(), bg.cuda(), scaling_modifier=0.6)
+        (out["render"] * r.image(dL, v)).sum().backward()
+        ok = ~amb[v].to(r.device)
+        err = (r.image(images, v) - out["render"].detach()).abs()[:, ok].max().item()
+        print(f"view {v}: max image error {err:.3g}", flush=True)
+        assert err < IMG_TOL
+    for k in RAW:
+        e = grad_rel_err(grads[k], getattr(m, k).grad)
+        assert e < GRAD_TOL, f"{k}: rel err {e}"
+
+
 def test_huge_splats_cover_every_cell():
     """Three faint Gaussians of standard deviation 2.5 at the centre cover
     every tile and cell of every view (hundreds of instances each, which the

@@ -139,18 +139,11 @@ def test_backward_matches_oracle(mode):
     assert e < GRAD_TOL
 
 
-def test_active_sh_degree_below_model_degree():
-    """Training raises the active SH degree step by step while the features
-    keep all 16 coefficients: with settings.sh_degree = 1 on a degree-3 model
-    the colours use the first 4 coefficients only (image against the oracle
-    evaluated at degree 1), every gradient matches the oracle, and the
-    coefficients above the active degree get exactly zero gradient."""
-    print("case: active SH degree 1 of 3", flush=True)
-    W, H, n = 96, 80, 2000
-    sc = _scene(n, seed=5)
-    cam = _cam(W, H)
-    bg = torch.tensor([0.3, 0.1, 0.6])
-    rs = settings_from_cam(cam, bg, 1)
+def _fwd_bwd_against_oracle(sc, cam, rs, seed):
+    """Forward and backward of scene ``sc`` under settings ``rs`` against the
+    oracle (image off threshold ties, every gradient); returns the device
+    gradients and the oracle image."""
+    W, H = rs.image_width, rs.image_height
     color, radii, state, _, _ = _run(sc, rs)
     ors = oracle_settings(rs)
     kw = dict(shs=sc["shs"], scales=sc["scales"], rotations=sc["rots"])
@@ -159,9 +152,7 @@ def test_active_sh_degree_below_model_degree():
     d = (color.cpu().double() - ocol).abs().numpy().max(0)
     print(f"  image: max |diff| {d[ok].max():.3e} on {int(ok.sum())} untied pixels")
     assert d[ok].max() < IMG_TOL
-    ocol3, _, _ = OR.rasterize(sc["means3D"], sc["opac"], oracle_settings(settings_from_cam(cam, bg, 3)), **kw)
-    assert (ocol3 - ocol).abs().max().item() > 1e-3  # degree 3 would give another image
-    g = torch.Generator().manual_seed(9)
+    g = torch.Generator().manual_seed(seed)
     dL = torch.randn(3, H, W, generator=g)
     dL[:, ost["ambiguous"]] = 0.0
     gg = RZ.backward_raw(state, radii, dL.cuda())
@@ -175,8 +166,39 @@ def test_active_sh_degree_below_model_degree():
         e = grad_rel_err(got, ref)
         print(f"  {nm}: rel err {e:.2e}")
         assert e < GRAD_TOL, f"{nm} gradient rel err {e}"
+    return gg, ocol
+
+
+def test_active_sh_degree_below_model_degree():
+    """Training raises the active SH degree step by step while the features
+    keep all 16 coefficients: with settings.sh_degree = 1 on a degree-3 model
+    the colours use the first 4 coefficients only (image against the oracle
+    evaluated at degree 1), every gradient matches the oracle, and the
+    coefficients above the active degree get exactly zero gradient."""
+    print("case: active SH degree 1 of 3", flush=True)
+    sc = _scene(2000, seed=5)
+    cam = _cam(96, 80)
+    bg = torch.tensor([0.3, 0.1, 0.6])
+    gg, ocol = _fwd_bwd_against_oracle(sc, cam, settings_from_cam(cam, bg, 1), seed=9)
+    kw = dict(shs=sc["shs"], scales=sc["scales"], rotations=sc["rots"])
+    ocol3, _, _ = OR.rasterize(sc["means3D"], sc["opac"], oracle_settings(settings_from_cam(cam, bg, 3)), **kw)
+    assert (ocol3 - ocol).abs().max().item() > 1e-3  # degree 3 would give another image
     assert torch.count_nonzero(gg["sh"][:, 4:]).item() == 0
     assert torch.count_nonzero(gg["sh"][:, 1:4]).item() > 0
+
+
+def test_scale_modifier_matches_oracle():
+    """The scaling modifier multiplies every scale before the 3D covariance
+    (the viewer's size slider): at 0.6 the image and every gradient match the
+    oracle, and the image differs from the unmodified one."""
+    print("case: scale modifier 0.6", flush=True)
+    sc = _scene(2000, seed=6)
+    cam = _cam(96, 80)
+    bg = torch.tensor([0.4, 0.4, 0.1])
+    _, ocol = _fwd_bwd_against_oracle(sc, cam, settings_from_cam(cam, bg, 3, scale_modifier=0.6), seed=10)
+    kw = dict(shs=sc["shs"], scales=sc["scales"], rotations=sc["rots"])
+    ocol1, _, _ = OR.rasterize(sc["means3D"], sc["opac"], oracle_settings(settings_from_cam(cam, bg, 3)), **kw)
+    assert (ocol1 - ocol).abs().max().item() > 1e-2
 
 
 def test_sort_pairs_stable():
