@@ -63,7 +63,9 @@ def pack_model(model, out=None):
 
 
 def unpack_grads(grads, sh_coeffs=16):
-    """(59, P) gradients -> dict shaped like the model's raw parameters."""
+    """(59, P) gradients -> dict shaped like the model's raw parameters;
+    ``sh_coeffs`` is the model's SH coefficients per channel,
+    (max_sh_degree + 1)^2 (16 for degree 3, 1 for degree 0)."""
     P = grads.shape[1]
     sh = grads[SH:SH + 3 * sh_coeffs].t().reshape(P, sh_coeffs, 3)
     return {
