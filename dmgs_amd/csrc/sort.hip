@@ -39,12 +39,16 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
                                                                   uint32_t* __restrict__ block_sums) {
   __shared__ uint32_t s_wave[kScanThreads / 64];
   const size_t base = (size_t)blockIdx.x * kScanTile;
+  // every load is issued before the first is used: offsets past n read the
+  // last value and count nothing (a guarded load would wait for each one)
+  const uint32_t* __restrict__ inb = in + base;
+  const uint32_t last = (uint32_t)min(n - base, (size_t)kScanTile) - 1u;
+  uint32_t v[kScanItems];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) v[i] = inb[min((uint32_t)(i * kScanThreads) + threadIdx.x, last)];
   uint32_t sum = 0;
 #pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const size_t idx = base + (size_t)i * kScanThreads + threadIdx.x;
-    if (idx < n) sum += in[idx];
-  }
+  for (int i = 0; i < kScanItems; ++i) sum += (uint32_t)(i * kScanThreads) + threadIdx.x <= last ? v[i] : 0u;
   uint32_t total;
   block_exclusive_scan<kScanThreads / 64>(sum, s_wave, total);
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
@@ -95,11 +99,18 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
   __shared__ uint32_t s_wave[kScanThreads / 64];
   __shared__ uint32_t tile[kScanTile + kScanTile / 32];
   const size_t base = (size_t)blockIdx.x * kScanTile;
+  // all loads in flight at once (offsets past n read the last value, stored as zero)
+  const uint32_t* __restrict__ inb = in + base;
+  const uint32_t last = (uint32_t)min(n - base, (size_t)kScanTile) - 1u;
+  {
+    uint32_t ld[kScanItems];
 #pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const int p = i * kScanThreads + (int)threadIdx.x;
-    const size_t idx = base + p;
-    tile[scan_pad(p)] = idx < n ? in[idx] : 0u;
+    for (int i = 0; i < kScanItems; ++i) ld[i] = inb[min((uint32_t)(i * kScanThreads) + threadIdx.x, last)];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const uint32_t o = (uint32_t)(i * kScanThreads) + threadIdx.x;
+      tile[scan_pad((int)o)] = o <= last ? ld[i] : 0u;
+    }
   }
   __syncthreads();
   uint32_t v[kScanItems];
