@@ -91,27 +91,46 @@ __device__ __forceinline__ int scan_pad(int p) { return p + (p >> 5); }
 // 4096-slot output chunk below K, the element whose [start, end) holds the
 // chunk's first slot (what chunk_first_kernel computes from the offsets).
 constexpr uint32_t kChunkSlots = 4096;
+// sums_raw: `spine` holds the raw block sums (at most kPrefixBlocks of them)
+// and each work-group adds up those before it, so no spine launch is needed;
+// otherwise `spine` holds their exclusive scan.
+constexpr int kPrefixItems = 8;
+constexpr size_t kPrefixBlocks = (size_t)kPrefixItems * kScanThreads;  // 2048 blocks: 8M values
 template <bool kExclusive>
 __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* __restrict__ in,
                                                                  uint32_t* __restrict__ out, size_t n,
-                                                                 const uint32_t* __restrict__ spine,
+                                                                 const uint32_t* __restrict__ spine, int sums_raw,
                                                                  uint32_t* __restrict__ chunk_first, uint32_t K) {
   __shared__ uint32_t s_wave[kScanThreads / 64];
   __shared__ uint32_t tile[kScanTile + kScanTile / 32];
   const size_t base = (size_t)blockIdx.x * kScanTile;
-  // all loads in flight at once (offsets past n read the last value, stored as zero)
+  // all loads in flight at once (offsets past n read the last value, stored as
+  // zero; block sums past this block's predecessors count nothing)
   const uint32_t* __restrict__ inb = in + base;
   const uint32_t last = (uint32_t)min(n - base, (size_t)kScanTile) - 1u;
+  uint32_t pre = 0;
   {
     uint32_t ld[kScanItems];
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) ld[i] = inb[min((uint32_t)(i * kScanThreads) + threadIdx.x, last)];
+    if (sums_raw) {
+      uint32_t ps[kPrefixItems];
+#pragma unroll
+      for (int i = 0; i < kPrefixItems; ++i) ps[i] = spine[min((uint32_t)(i * kScanThreads) + threadIdx.x, gridDim.x - 1)];
+#pragma unroll
+      for (int i = 0; i < kPrefixItems; ++i) pre += (uint32_t)(i * kScanThreads) + threadIdx.x < blockIdx.x ? ps[i] : 0u;
+    }
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
       const uint32_t o = (uint32_t)(i * kScanThreads) + threadIdx.x;
       tile[scan_pad((int)o)] = o <= last ? ld[i] : 0u;
     }
   }
+  uint32_t prefix;
+  if (sums_raw)
+    block_exclusive_scan<kScanThreads / 64>(pre, s_wave, prefix);  // its barriers also order the tile stores
+  else
+    prefix = spine[blockIdx.x];
   __syncthreads();
   uint32_t v[kScanItems];
   uint32_t tsum = 0;
@@ -121,7 +140,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t
     tsum += v[i];
   }
   uint32_t total;
-  uint32_t run = spine[blockIdx.x] + block_exclusive_scan<kScanThreads / 64>(tsum, s_wave, total);  // ends with a barrier
+  uint32_t run = prefix + block_exclusive_scan<kScanThreads / 64>(tsum, s_wave, total);  // ends with a barrier
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
     const uint32_t x = v[i];
@@ -151,13 +170,14 @@ static void scan_impl(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scr
   if (n == 0) return;
   const size_t nb = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, scratch);
-  hipLaunchKernelGGL(scan_spine_kernel, dim3(1), dim3(kScanThreads), 0, s, scratch, nb);
+  const int sums_raw = nb <= kPrefixBlocks ? 1 : 0;  // few blocks: each adds up its own prefix
+  if (!sums_raw) hipLaunchKernelGGL(scan_spine_kernel, dim3(1), dim3(kScanThreads), 0, s, scratch, nb);
   if (exclusive)
     hipLaunchKernelGGL(scan_apply_kernel<true>, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n,
-                       (const uint32_t*)scratch, (uint32_t*)nullptr, 0u);
+                       (const uint32_t*)scratch, sums_raw, (uint32_t*)nullptr, 0u);
   else
     hipLaunchKernelGGL(scan_apply_kernel<false>, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, out, n,
-                       (const uint32_t*)scratch, chunk_first, K);
+                       (const uint32_t*)scratch, sums_raw, chunk_first, K);
 }
 
 void launch_inclusive_scan_chunks(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch,
@@ -776,7 +796,7 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   if (n) hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
 }
 
-// Four sorted keys per thread (one 16-byte load), each compared with its
+// Sixteen sorted keys per thread (four 16-byte loads in flight), each compared with its
 // predecessor: a key that differs from the previous one closes the previous
 // tile's range and opens its own.  With views, a masked key is
 // (cell within the view | view << 12) and indexes ranges through the view's
@@ -785,26 +805,31 @@ __device__ __forceinline__ uint32_t range_slot(uint32_t t, const ViewDesc* views
   return views ? (uint32_t)views[t >> 12].cell_base + (t & 0xFFFu) : t;
 }
 
+constexpr int kRangeKeys = 16;  // sorted keys per thread of tile_ranges_kernel
+
 __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, uint2* __restrict__ ranges,
                                    uint32_t num_tiles, const uint32_t* __restrict__ K_dev, uint32_t key_mask,
                                    const ViewDesc* __restrict__ views) {
   if (K_dev) K = min(K, (size_t)*K_dev);
-  const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  const size_t i0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * kRangeKeys;
   if (i0 >= K) return;
-  uint32_t k[4];
-  if (i0 + 4 <= K) {
-    const uint4 q = *reinterpret_cast<const uint4*>(keys + i0);
-    k[0] = q.x & key_mask;
-    k[1] = q.y & key_mask;
-    k[2] = q.z & key_mask;
-    k[3] = q.w & key_mask;
+  uint32_t k[kRangeKeys];
+  if (i0 + kRangeKeys <= K) {  // every thread but the last: four 16-byte loads in flight
+#pragma unroll
+    for (int q = 0; q < kRangeKeys / 4; ++q) {
+      const uint4 v = reinterpret_cast<const uint4*>(keys + i0)[q];
+      k[4 * q] = v.x & key_mask;
+      k[4 * q + 1] = v.y & key_mask;
+      k[4 * q + 2] = v.z & key_mask;
+      k[4 * q + 3] = v.w & key_mask;
+    }
   } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) k[j] = (i0 + j < K) ? keys[i0 + j] & key_mask : 0u;
+    for (int j = 0; j < kRangeKeys; ++j) k[j] = (i0 + j < K) ? keys[i0 + j] & key_mask : 0u;
   }
   uint32_t prev = (i0 == 0) ? 0u : keys[i0 - 1] & key_mask;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < kRangeKeys; ++j) {
     const size_t i = i0 + j;
     if (i >= K) break;
     // boundaries are rare: only they look up range slots (a table read with views)
@@ -830,7 +855,7 @@ __global__ void tile_ranges_kernel(const uint32_t* __restrict__ keys, size_t K, 
 void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int num_tiles, hipStream_t s,
                         const uint32_t* K_dev, uint32_t key_mask) {
   launch_fill_u32((uint32_t*)ranges, (size_t)num_tiles * 2, 0u, s);
-  const size_t threads = (K + 3) / 4;
+  const size_t threads = (K + kRangeKeys - 1) / kRangeKeys;
   if (K) hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, tile_keys,
                             K, ranges, (uint32_t)num_tiles, K_dev, key_mask, (const ViewDesc*)nullptr);
 }
@@ -838,7 +863,7 @@ void launch_tile_ranges(const uint32_t* tile_keys, size_t K, uint2* ranges, int 
 void launch_view_cell_ranges(const uint32_t* keys, size_t K, uint2* ranges, int num_cells, const ViewDesc* views,
                              hipStream_t s, const uint32_t* K_dev, bool ranges_zeroed) {
   if (!ranges_zeroed) launch_fill_u32((uint32_t*)ranges, (size_t)num_cells * 2, 0u, s);
-  const size_t threads = (K + 3) / 4;
+  const size_t threads = (K + kRangeKeys - 1) / kRangeKeys;
   if (K) hipLaunchKernelGGL(tile_ranges_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, keys, K,
                             ranges, (uint32_t)num_cells, K_dev, (uint32_t)DMGS_CELL_KEY_MASK, views);
 }
