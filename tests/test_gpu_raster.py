@@ -226,7 +226,9 @@ def test_sort_pairs_stable():
 def test_inclusive_scan():
     from dmgs_amd import _native as N
     rng = np.random.default_rng(1)
-    for n in [1, 255, 4096, 4097, 1_000_003]:
+    # up to 2048 blocks of 4096 each block adds up its own prefix; past that
+    # a spine launch scans the block sums
+    for n in [1, 255, 4096, 4097, 1_000_003, 2048 * 4096, 2048 * 4096 + 12_345]:
         print(f"case: scan n={n}", flush=True)
         x = rng.integers(0, 50, size=n, dtype=np.uint32)
         xt = torch.from_numpy(x.view(np.int32)).cuda()
