@@ -295,29 +295,53 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(const uint32_t
 
 // Exclusive scan of row `blockIdx.x` (one digit, nb block counts) in place;
 // the row total goes to totals[digit].
-__global__ __launch_bounds__(kScanThreads) void radix_rowscan_kernel(uint32_t* __restrict__ hist, uint32_t nb,
-                                                                    uint32_t* __restrict__ totals) {
-  __shared__ uint32_t s_wave[kScanThreads / 64];
+// NT threads hold NT x 16 block counts per round (512 when a row has more
+// than 4096 blocks, so the cell sort's rows are scanned in one round).  The
+// row moves through LDS like scan_apply_kernel's tile: consecutive lanes read
+// and write consecutive counts, and each thread scans 16 consecutive ones.
+template <int NT>
+__global__ __launch_bounds__(NT) void radix_rowscan_kernel(uint32_t* __restrict__ hist, uint32_t nb,
+                                                          uint32_t* __restrict__ totals) {
+  constexpr int kTile = NT * kScanItems;
+  __shared__ uint32_t s_wave[NT / 64];
+  __shared__ uint32_t tile[kTile + kTile / 32];
   uint32_t* row = hist + (size_t)blockIdx.x * nb;
   uint32_t carry = 0;
-  for (uint32_t base = 0; base < nb; base += kScanTile) {
+  for (uint32_t base = 0; base < nb; base += (uint32_t)kTile) {
+    const uint32_t last = min(nb - base, (uint32_t)kTile) - 1u;
+    {
+      uint32_t ld[kScanItems];
+#pragma unroll
+      for (int i = 0; i < kScanItems; ++i) ld[i] = row[base + min((uint32_t)(i * NT) + threadIdx.x, last)];
+#pragma unroll
+      for (int i = 0; i < kScanItems; ++i) {
+        const uint32_t o = (uint32_t)(i * NT) + threadIdx.x;
+        tile[scan_pad((int)o)] = o <= last ? ld[i] : 0u;
+      }
+    }
+    __syncthreads();
     uint32_t v[kScanItems];
     uint32_t tsum = 0;
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-      const uint32_t idx = base + threadIdx.x * kScanItems + i;
-      v[i] = idx < nb ? row[idx] : 0u;
+      v[i] = tile[scan_pad((int)threadIdx.x * kScanItems + i)];
       tsum += v[i];
     }
     uint32_t total;
-    uint32_t run = carry + block_exclusive_scan<kScanThreads / 64>(tsum, s_wave, total);
+    uint32_t run = carry + block_exclusive_scan<NT / 64>(tsum, s_wave, total);  // ends with a barrier
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-      const uint32_t idx = base + threadIdx.x * kScanItems + i;
-      if (idx < nb) row[idx] = run;
+      tile[scan_pad((int)threadIdx.x * kScanItems + i)] = run;
       run += v[i];
     }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const uint32_t o = (uint32_t)(i * NT) + threadIdx.x;
+      if (o <= last) row[base + o] = tile[scan_pad((int)o)];
+    }
     carry += total;
+    __syncthreads();  // the tile is read out before the next round fills it
   }
   if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
@@ -452,6 +476,13 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
   }
 }
 
+static void launch_rowscan(uint32_t* hist, uint32_t nb, uint32_t* totals, uint32_t digits, hipStream_t s) {
+  if (nb > (uint32_t)kScanTile)
+    hipLaunchKernelGGL(radix_rowscan_kernel<512>, dim3(digits), dim3(512), 0, s, hist, nb, totals);
+  else
+    hipLaunchKernelGGL(radix_rowscan_kernel<kScanThreads>, dim3(digits), dim3(kScanThreads), 0, s, hist, nb, totals);
+}
+
 static int sort_items(size_t n) { return n >= ((size_t)4 << 20) ? 16 : 4; }
 
 size_t radix_scratch_words(size_t n) {
@@ -490,7 +521,7 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
     if (p == 0 && entry) {
       hipLaunchKernelGGL((radix_hist_kernel<ITEMS, true, true>), dim3(nb), dim3(kSortThreads), 0, s,
                          (const uint32_t*)ka, n, shift, w, hist, nb, (const uint32_t*)nullptr, entry->flags);
-      hipLaunchKernelGGL(radix_rowscan_kernel, dim3(1u << w), dim3(kScanThreads), 0, s, hist, nb, totals);
+      launch_rowscan(hist, nb, totals, 1u << w, s);
       hipLaunchKernelGGL((radix_scatter_kernel<ITEMS, true>), dim3(nb), dim3(kSortThreads), 0, s,
                          (const uint32_t*)ka, (const uint32_t*)nullptr, kb, vb, n, shift, w, (const uint32_t*)hist,
                          (const uint32_t*)totals, nb, (const uint32_t*)nullptr, entry->n_out);
@@ -508,7 +539,7 @@ static void radix_passes(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
       hipLaunchKernelGGL((radix_hist_kernel<ITEMS, false>), dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka,
                          n, shift, w, hist, nb, n_dev);
     }
-    hipLaunchKernelGGL(radix_rowscan_kernel, dim3(1u << w), dim3(kScanThreads), 0, s, hist, nb, totals);
+    launch_rowscan(hist, nb, totals, 1u << w, s);
     hipLaunchKernelGGL(radix_scatter_kernel<ITEMS>, dim3(nb), dim3(kSortThreads), 0, s, (const uint32_t*)ka,
                        (const uint32_t*)va, kb, vb, n, shift, w, (const uint32_t*)hist, (const uint32_t*)totals, nb,
                        n_dev);
