@@ -361,9 +361,20 @@ __global__ __launch_bounds__(256) void dssim_reduce_kernel(const float* __restri
   partials += (size_t)2 * nblocks * blockIdx.x;
   out += 3 * blockIdx.x;
   float s = 0.f, l = 0.f;
-  for (int i = threadIdx.x; i < nblocks; i += 256) {
-    s += partials[2 * i];
-    l += partials[2 * i + 1];
+  // eight pairs per thread requested at once (indices past the end read the
+  // last pair and add nothing); each thread still adds its pairs in order
+  const float2* __restrict__ pp = reinterpret_cast<const float2*>(partials);
+  for (int i0 = threadIdx.x; i0 < nblocks; i0 += 256 * 8) {
+    float2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = pp[min(i0 + 256 * u, nblocks - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (i0 + 256 * u < nblocks) {
+        s += v[u].x;
+        l += v[u].y;
+      }
+    }
   }
   s = wave_sum(s);
   l = wave_sum(l);
