@@ -199,9 +199,12 @@ __device__ __forceinline__ void load_sh16(const float* __restrict__ sh, int M, f
       r[4 * q + 2] = v.z;
       r[4 * q + 3] = v.w;
     }
-  } else {
+  } else {  // fewer coefficients: every load issued at once (indices past 3M read the last one, then zero)
+    const int last = 3 * min(M, 16) - 1;
 #pragma unroll
-    for (int i = 0; i < 48; ++i) r[i] = (i / 3 < M) ? sh[i] : 0.f;
+    for (int i = 0; i < 48; ++i) r[i] = last >= 0 ? sh[min(i, last)] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 48; ++i) r[i] = (i / 3 < M) ? r[i] : 0.f;
   }
 }
 
